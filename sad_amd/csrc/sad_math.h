@@ -1,0 +1,377 @@
+/*
+ * sad_math.h - types and exact arithmetic primitives of the SAD fit/render path.
+ *
+ * This header is plain C99 and HIP at the same time: every function is marked
+ * SAD_HD, which is `__host__ __device__` under hipcc and `static inline` under a
+ * C compiler.  The candidate search (JFA flood, VPT) must produce bit-identical
+ * candidate lists on the GPU and on a CPU, so every floating-point operation it
+ * performs comes from this file, is written with explicit fmaf where a fused
+ * multiply-add is wanted, and is compiled with -ffp-contract=off on both sides.
+ * IEEE division and sqrtf are correctly rounded on both targets (hipcc lowers
+ * fp32 `/` and sqrtf to the div_scale/div_fmas/div_fixup and corrected
+ * v_sqrt sequences), so they are used directly.
+ *
+ * Site model (reference: backends/cuda/src/sad_common.cuh:21-58):
+ *   Site       10 fp32 = pos.xy, log_tau, radius, rgb, aniso_dir.xy, log_aniso (40 B)
+ *   AdamState  20 fp32 = m/v per parameter group, pos and dir share one v (80 B).
+ *              The CUDA struct is 88 B because float2 members are 8-byte aligned;
+ *              the torch op contract allocates [N,20] (backends/pytorch/train.py:836,918),
+ *              so this build packs the 18 state floats without the alignment hole.
+ */
+#ifndef SAD_MATH_H
+#define SAD_MATH_H
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SAD_HD static __host__ __device__ __forceinline__
+#else
+#define SAD_HD static inline
+#endif
+
+#include "../../include/sad_types.h"
+
+/* ---------------------------------------------------------------- bits -- */
+
+SAD_HD uint32_t sad_f2u(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+SAD_HD float sad_u2f(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+
+SAD_HD int sad_clampi(int x, int a, int b) { return x < a ? a : (x > b ? b : x); }
+SAD_HD float sad_clampf(float x, float a, float b) { return fmaxf(a, fminf(b, x)); }
+
+/* ----------------------------------------------------------- exp / log --
+ * sad_expf: Cody & Waite (1980) argument reduction x = k*ln2 + r, |r| <= ln2/2,
+ * with ln2 split into a 16-bit head and a tail, followed by the degree-7 Taylor
+ * polynomial of e^r in Horner form (Muller, "Elementary Functions", 3rd ed.,
+ * ch. 11).  Truncation error r^8/8! < 8e-9, so the result is within ~1 ulp of
+ * e^x.  2^k is applied in two exact power-of-two steps so that gradual
+ * underflow rounds once.  Only +, *, fmaf and integer operations are used, so
+ * the result is bit-identical on every IEEE binary32 target. */
+SAD_HD float sad_pow2i(int k) { return sad_u2f((uint32_t)(k + 127) << 23); }
+
+SAD_HD float sad_expf(float x)
+{
+    if (!(x == x)) return x + x;                 /* NaN */
+    if (x > 0x1.62e430p+6f) return INFINITY;
+    if (x < -103.97208404541015625f) return 0.0f;
+    const float L2E = 0x1.715476p+0f;   /* log2(e) */
+    const float LN2_HI = 0x1.62e400p-1f;   /* ln2 head, 16 significant bits */
+    const float LN2_LO = 0x1.7f7d1cp-20f;  /* ln2 - head */
+    const float SHIFTER = 12582912.0f;                     /* 1.5 * 2^23 */
+    float t = x * L2E;
+    float kf = (t + SHIFTER) - SHIFTER;          /* round-to-nearest-even of t */
+    float r = fmaf(-kf, LN2_HI, x);
+    r = fmaf(-kf, LN2_LO, r);
+    float p = 0x1.a01a02p-13f;   /* 1/5040 */
+    p = fmaf(p, r, 0x1.6c16c2p-10f);  /* 1/720 */
+    p = fmaf(p, r, 0x1.111112p-7f);   /* 1/120 */
+    p = fmaf(p, r, 0x1.555556p-5f);   /* 1/24 */
+    p = fmaf(p, r, 0x1.555556p-3f);   /* 1/6 */
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    int k = (int)kf;
+    int k1 = k / 2;
+    int k2 = k - k1;
+    return (p * sad_pow2i(k1)) * sad_pow2i(k2);
+}
+
+/* sad_logf (x > 0): x = m * 2^e with m in [sqrt(1/2), sqrt(2)), then
+ * log(m) = 2 atanh(s), s = (m-1)/(m+1), summed as s + s^3/3 + ... + s^11/11
+ * (|s| < 0.1716, truncation < 2e-10 relative).  Used for pow() in the densify
+ * score, where CPU and GPU must agree bit for bit. */
+SAD_HD float sad_logf(float x)
+{
+    if (!(x == x)) return x + x;
+    if (x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return INFINITY;
+    uint32_t u = sad_f2u(x);
+    int e = 0;
+    if (u < 0x00800000u) {                       /* subnormal: scale by 2^24 */
+        x = x * 16777216.0f;
+        u = sad_f2u(x);
+        e = -24;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = sad_u2f((u & 0x007FFFFFu) | 0x3F800000u);    /* [1,2) */
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float q = 0x1.745d18p-4f;   /* 1/11 */
+    q = fmaf(q, s2, 0x1.c71c72p-4f);  /* 1/9 */
+    q = fmaf(q, s2, 0x1.24924ap-3f);  /* 1/7 */
+    q = fmaf(q, s2, 0x1.99999ap-3f);  /* 1/5 */
+    q = fmaf(q, s2, 0x1.555556p-2f);  /* 1/3 */
+    float lm = 2.0f * fmaf(s * s2, q, s);
+    const float LN2_HI = 0x1.62e400p-1f;
+    const float LN2_LO = 0x1.7f7d1cp-20f;
+    float ef = (float)e;
+    return fmaf(ef, LN2_HI, fmaf(ef, LN2_LO, lm));
+}
+
+/* x^a for x > 0 (pow of a positive base), as exp(a * log x). */
+SAD_HD float sad_powf_pos(float x, float a) { return sad_expf(a * sad_logf(x)); }
+
+/* sin/cos for |a| < 2^10: Cody-Waite reduction by pi/2 (head + tail of pi/2),
+ * then the Taylor polynomials of sin (degree 9) and cos (degree 8) on
+ * |r| <= pi/4 (truncation < 4e-9) and the quadrant identities.  Used by the VPT
+ * ring probes (jfa.cu:345-358), whose offsets must match between CPU and GPU. */
+SAD_HD void sad_sincosf(float a, float* s_out, float* c_out)
+{
+    const float TWO_OVER_PI = 0x1.45f306p-1f;
+    const float PIO2_HI = 0x1.921fb6p+0f;
+    const float PIO2_LO = -0x1.777a5cp-25f;
+    const float SHIFTER = 12582912.0f;
+    float t = a * TWO_OVER_PI;
+    float qf = (t + SHIFTER) - SHIFTER;
+    float r = fmaf(-qf, PIO2_HI, a);
+    r = fmaf(-qf, PIO2_LO, r);
+    float r2 = r * r;
+    float sp = -0x1.a01a02p-13f;                  /* -1/5040 */
+    sp = fmaf(sp, r2, 0x1.111112p-7f);            /* 1/120 */
+    sp = fmaf(sp, r2, -0x1.555556p-3f);           /* -1/6 */
+    sp = fmaf(sp * r2, r, r);
+    float sp9 = 0x1.71de3ap-19f * (r2 * r2) * (r2 * r2) * r;   /* r^9/9! */
+    float sn = sp + sp9;
+    float cp = 0x1.a01a02p-16f;                   /* 1/40320 */
+    cp = fmaf(cp, r2, -0x1.6c16c2p-10f);          /* -1/720 */
+    cp = fmaf(cp, r2, 0x1.555556p-5f);            /* 1/24 */
+    cp = fmaf(cp, r2, -0.5f);
+    float cs = fmaf(cp, r2, 1.0f);
+    int q = ((int)qf) & 3;
+    float s, c;
+    if (q == 0) { s = sn; c = cs; }
+    else if (q == 1) { s = cs; c = -sn; }
+    else if (q == 2) { s = -sn; c = -cs; }
+    else { s = -cs; c = sn; }
+    *s_out = s;
+    *c_out = c;
+}
+SAD_HD float sad_sinf(float a) { float s, c; sad_sincosf(a, &s, &c); return s; }
+SAD_HD float sad_cosf(float a) { float s, c; sad_sincosf(a, &s, &c); return c; }
+
+/* ------------------------------------------------------------- fp16 --
+ * IEEE binary32 -> binary16, round to nearest even (IEEE 754-2008 5.4.2),
+ * including binary16 subnormals; NaN payloads are kept quiet. */
+SAD_HD uint16_t sad_f32_to_f16(float f)
+{
+    uint32_t x = sad_f2u(f);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) {
+        if (ax > 0x7F800000u) return (uint16_t)(sign | 0x7E00u | ((ax >> 13) & 0x3FFu));
+        return (uint16_t)(sign | 0x7C00u);
+    }
+    if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);      /* >= 65520 -> inf */
+    if (ax < 0x38800000u) {                                         /* < 2^-14 */
+        if (ax <= 0x33000000u) return (uint16_t)sign;               /* <= 2^-25 -> 0 */
+        uint32_t e = ax >> 23;
+        uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+        uint32_t shift = 126u - e;
+        uint32_t hm = m >> shift;
+        uint32_t rem = m & ((1u << shift) - 1u);
+        uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (hm & 1u))) hm += 1u;
+        return (uint16_t)(sign | hm);
+    }
+    uint32_t h = (ax - 0x38000000u) >> 13;
+    uint32_t rem = ax & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;
+    return (uint16_t)(sign | h);
+}
+
+SAD_HD float sad_f16_to_f32(uint16_t h)
+{
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t e = ((uint32_t)h >> 10) & 0x1Fu;
+    uint32_t m = (uint32_t)h & 0x3FFu;
+    if (e == 0u) {
+        float v = (float)m * 5.9604644775390625e-08f;   /* m * 2^-24, exact */
+        return sign ? -v : v;
+    }
+    if (e == 31u) return sad_u2f(sign | 0x7F800000u | (m << 13));
+    return sad_u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+/* ------------------------------------------------------------ hashes -- */
+
+/* xorshift32 with the (13,17,5) triple, sad_common.cuh:128-133 */
+SAD_HD uint32_t sad_xorshift32(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+/* Knuth multiplicative hash used for the tile hash, sad_common.cuh:168-170 */
+SAD_HD uint32_t sad_tile_hash(uint32_t key) { return key * 2654435761u; }
+
+/* Hilbert curve index of (x, y) on a 2^bits grid, sad_cuda.cu:1523-1545 */
+SAD_HD uint32_t sad_hilbert_index(uint32_t x, uint32_t y, int bits)
+{
+    uint32_t xi = x, yi = y, index = 0;
+    uint32_t mask = (bits >= 32) ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    for (int i = bits - 1; i >= 0; --i) {
+        uint32_t shift = (uint32_t)i;
+        uint32_t rx = (xi >> shift) & 1u;
+        uint32_t ry = (yi >> shift) & 1u;
+        index |= ((3u * rx) ^ ry) << (2u * shift);
+        if (ry == 0u) {
+            if (rx == 1u) { xi = mask - xi; yi = mask - yi; }
+            uint32_t t = xi; xi = yi; yi = t;
+        }
+    }
+    return index;
+}
+
+/* candidate cell of pixel (x, y), sad_common.cuh:113-125 */
+SAD_HD int sad_cand_index(int x, int y, int outW, int outH, int candW, int candH)
+{
+    int sw = outW > 0 ? outW : 1;
+    int sh = outH > 0 ? outH : 1;
+    int cx = sad_clampi((x * candW) / sw, 0, candW - 1);
+    int cy = sad_clampi((y * candH) / sh, 0, candH - 1);
+    return cy * candW + cx;
+}
+
+/* ------------------------------------------------------ soft distance --
+ * voronoi_dmix2 (sad_common.cuh:141-165): anisotropic det-1 metric
+ * G = e^a u u^T + e^-a (I - u u^T); returns sqrt(max(d^T G d / S^2, 1e-8)) - r/S.
+ * The per-site terms l1 = e^a, l2 = 1/l1 and r/S are passed in precomputed so a
+ * caller can hoist them out of a per-pixel loop; sad_dmix2 computes them the
+ * same way, so both forms give identical bits. */
+SAD_HD float sad_dmix2_pre(float px, float py, float dirx, float diry,
+                           float l1, float l2, float r_norm,
+                           float ux, float uy, float inv_scale_sq)
+{
+    float dx = ux - px;
+    float dy = uy - py;
+    float diff2 = fmaf(dx, dx, dy * dy);
+    float proj = fmaf(dirx, dx, diry * dy);
+    float proj2 = proj * proj;
+    float perp2 = fmaxf(diff2 - proj2, 0.0f);
+    float d2a = fmaf(l1, proj2, l2 * perp2);
+    float d2s = fmaxf(d2a * inv_scale_sq, 1e-8f);
+    return sqrtf(d2s) - r_norm;
+}
+
+SAD_HD float sad_dmix2(float px, float py, float dirx, float diry, float log_aniso,
+                       float radius, float ux, float uy, float inv_scale_sq)
+{
+    float l1 = sad_expf(log_aniso);
+    float l2 = 1.0f / l1;
+    float r_norm = radius * sqrtf(inv_scale_sq);
+    return sad_dmix2_pre(px, py, dirx, diry, l1, l2, r_norm, ux, uy, inv_scale_sq);
+}
+
+/* Per-site terms of the candidate-search key, hoisted out of the per-cell loop:
+ * the key tau * dmix (jfa.cu:33-35,171-173) is tau * sad_dmix2_pre(..., l1, l2, rn)
+ * with tau = max(e^log_tau, 1e-4), l1 = e^log_aniso, l2 = 1/l1,
+ * rn = radius * inv_scale and inv_scale = sqrtf(inv_scale_sq).  pos_x < 0 marks
+ * an inactive site (key +inf). */
+typedef struct SadSearchSite {
+    float px, py, dx, dy;
+    float l1, l2, rn, tau;
+} SadSearchSite;
+
+SAD_HD SadSearchSite sad_search_site(float px, float py, float log_tau, float radius,
+                                     float dx, float dy, float log_aniso, float inv_scale)
+{
+    SadSearchSite s;
+    s.px = px; s.py = py; s.dx = dx; s.dy = dy;
+    s.l1 = sad_expf(log_aniso);
+    s.l2 = 1.0f / s.l1;
+    s.rn = radius * inv_scale;
+    s.tau = fmaxf(sad_expf(log_tau), 1e-4f);
+    return s;
+}
+
+SAD_HD float sad_search_key(SadSearchSite s, float ux, float uy, float inv_scale_sq)
+{
+    if (s.px < 0.0f) return INFINITY;
+    return s.tau * sad_dmix2_pre(s.px, s.py, s.dx, s.dy, s.l1, s.l2, s.rn, ux, uy, inv_scale_sq);
+}
+
+/* Candidate-search key of a full-precision site (JFA flood, jfa.cu:33-35).
+ * Callers have already skipped inactive sites. */
+SAD_HD float sad_site_key(const SadSite* s, float ux, float uy, float inv_scale_sq)
+{
+    float d = sad_dmix2(s->pos_x, s->pos_y, s->dir_x, s->dir_y, s->log_aniso, s->radius,
+                        ux, uy, inv_scale_sq);
+    float tau = fmaxf(sad_expf(s->log_tau), 1e-4f);
+    return tau * d;
+}
+
+/* Same key from the fp16-packed site; inactive (pos.x < 0) -> +inf (jfa.cu:150-174). */
+SAD_HD float sad_packed_key(SadPackedSite p, float ux, float uy, float inv_scale_sq)
+{
+    float px = sad_f16_to_f32(p.h[0]);
+    if (px < 0.0f) return INFINITY;
+    float py = sad_f16_to_f32(p.h[1]);
+    float lt = sad_f16_to_f32(p.h[2]);
+    float rad = sad_f16_to_f32(p.h[3]);
+    float dx = sad_f16_to_f32(p.h[4]);
+    float dy = sad_f16_to_f32(p.h[5]);
+    float la = sad_f16_to_f32(p.h[6]);
+    float d = sad_dmix2(px, py, dx, dy, la, rad, ux, uy, inv_scale_sq);
+    float tau = fmaxf(sad_expf(lt), 1e-4f);
+    return tau * d;
+}
+
+SAD_HD SadPackedSite sad_pack_site(const SadSite* s)
+{
+    SadPackedSite p;
+    if (s->pos_x < 0.0f) {
+        p.h[0] = sad_f32_to_f16(-1.0f);
+        p.h[1] = sad_f32_to_f16(-1.0f);
+        for (int i = 2; i < 8; ++i) p.h[i] = 0;
+    } else {
+        p.h[0] = sad_f32_to_f16(s->pos_x);
+        p.h[1] = sad_f32_to_f16(s->pos_y);
+        p.h[2] = sad_f32_to_f16(s->log_tau);
+        p.h[3] = sad_f32_to_f16(s->radius);
+        p.h[4] = sad_f32_to_f16(s->dir_x);
+        p.h[5] = sad_f32_to_f16(s->dir_y);
+        p.h[6] = sad_f32_to_f16(s->log_aniso);
+        p.h[7] = 0;
+    }
+    return p;
+}
+
+/* ------------------------------------------------------ jump schedule --
+ * VPT jump step for pass index i: pow2 = next power of two >= max(w,h),
+ * stage = min(i, log2(pow2) - 1), step = max(pow2 >> (stage+1), 1)
+ * (sad_cuda.cu:1157-1183, candidate_field.swift:10-33). */
+SAD_HD uint32_t sad_jump_step(uint32_t step_index, int width, int height)
+{
+    uint32_t max_dim = (uint32_t)(width > height ? width : height);
+    uint32_t pow2 = 1u;
+    while (pow2 < max_dim) pow2 <<= 1;
+    if (pow2 <= 1u) return 1u;
+    uint32_t stages = 0u, t = pow2;
+    while (t > 1u) { t >>= 1; stages += 1u; }
+    uint32_t stage = step_index >= stages ? stages - 1u : step_index;
+    uint32_t step = pow2 >> (stage + 1u);
+    return step > 1u ? step : 1u;
+}
+
+SAD_HD uint32_t sad_pack_jump_step(uint32_t step_index, int width, int height)
+{
+    uint32_t j = sad_jump_step(step_index, width, height);
+    if (j > 0xFFFFu) j = 0xFFFFu;
+    return (j << 16) | (step_index & 0xFFFFu);
+}
+
+SAD_HD int sad_hilbert_bits(int width, int height)
+{
+    int max_dim = width > height ? width : height;
+    int n = 1, bits = 0;
+    while (n < max_dim) { n <<= 1; bits += 1; }
+    return bits > 1 ? bits : 1;
+}
+
+#endif /* SAD_MATH_H */
