@@ -244,6 +244,8 @@ typedef struct SadhFitResult {
     double grad_kernel_ms;            /* summed gradient-kernel time (profile_gradients) */
     int grad_launches;
     double grad_bytes;                /* summed algorithmic bytes of those launches */
+    float* grad_ms_trace;             /* in: optional host array; out: per-launch gradient ms */
+    int grad_ms_trace_cap;            /* in: entries of grad_ms_trace */
 } SadhFitResult;
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does

@@ -312,8 +312,9 @@ ORC_API void orc_update_candidates(const uint32_t* in0, const uint32_t* in1,
 /* -------------------------------------------------------------- gradients --
  * computeGradientsTiledKernel, gradients.cu:103-486 (L2 loss; the SSIM branch
  * is not restated: ssim_weight = 0 on this path).  The per-candidate analytic
- * derivative follows gradients.cu:352-420; the tile hash is an accumulation
- * device only, so the oracle sums per site directly.
+ * derivative of gradients.cu:352-420 is sad_grad_terms (sad_math.h), which is
+ * pinned separately against central finite differences of the render; the
+ * tile hash is an accumulation device only, so the oracle sums per site directly.
  * acc[10*id + j] (double) receives the sum of contributions for parameter j in
  * Site field order {pos.x, pos.y, log_tau, radius, r, g, b, dir.x, dir.y,
  * log_aniso}; abs (nullable) receives the sum of |contribution|.  rdelta/rabs
@@ -321,49 +322,16 @@ ORC_API void orc_update_candidates(const uint32_t* in0, const uint32_t* in1,
 static void orc_grad_candidate(const SadSite* s, float w, const float g[3], const float pred[3],
                                float ux, float uy, float iss, float out[10])
 {
-    out[4] = w * g[0];
-    out[5] = w * g[1];
-    out[6] = w * g[2];
-    float dlogit = g[0] * w * (s->col_r - pred[0]) + g[1] * w * (s->col_g - pred[1])
-                 + g[2] * w * (s->col_b - pred[2]);
-    float tau = sad_expf(s->log_tau);
-    float dx = ux - s->pos_x, dy = uy - s->pos_y;
-    float diff2 = dx * dx + dy * dy;
-    float proj = s->dir_x * dx + s->dir_y * dy;
-    float proj2 = proj * proj;
-    float perp2 = fmaxf(diff2 - proj2, 0.0f);
     float l1 = sad_expf(s->log_aniso);
     float l2 = 1.0f / fmaxf(l1, 1e-8f);
-    float d2n = (l1 * proj2 + l2 * perp2) * iss;
-    float d2s = fmaxf(d2n, 1e-8f);
-    float isd = 1.0f / sqrtf(d2s);
-    float inv_scale = sqrtf(iss);
-    float dmix = sqrtf(d2s) - s->radius * inv_scale;
-    float gx = l2 * dx + (l1 - l2) * s->dir_x * proj;
-    float gy = l2 * dy + (l1 - l2) * s->dir_y * proj;
-    float cpos = dlogit * (tau * iss * isd);
-    out[0] = cpos * gx;
-    out[1] = cpos * gy;
-    out[2] = dlogit * (-dmix) * tau;
-    out[3] = dlogit * tau * inv_scale;
-    float half_isd = 0.5f * isd;
-    float dla = (l1 * proj2 - l2 * perp2) * iss;
-    out[9] = dlogit * (-tau) * half_isd * dla;
-    float cd = 2.0f * (l1 - l2) * proj * iss;
-    out[7] = dlogit * (-tau) * half_isd * (cd * dx);
-    out[8] = dlogit * (-tau) * half_isd * (cd * dy);
+    sad_grad_terms(s->pos_x, s->pos_y, sad_expf(s->log_tau), s->radius * sqrtf(iss), s->dir_x, s->dir_y, l1, l2,
+                   s->col_r, s->col_g, s->col_b, ux, uy, w, g[0], g[1], g[2], pred[0], pred[1], pred[2], iss,
+                   sqrtf(iss), out);
 }
 
 static float orc_removal_delta(const float pred[3], const float tgt[3], const SadSite* s, float wt)
 {
-    float keep = 1.0f - wt;
-    if (keep < 1e-4f) return 1.0e6f;
-    float pk0 = (pred[0] - wt * s->col_r) / keep;
-    float pk1 = (pred[1] - wt * s->col_g) / keep;
-    float pk2 = (pred[2] - wt * s->col_b) / keep;
-    float d0 = pred[0] - tgt[0], d1 = pred[1] - tgt[1], d2 = pred[2] - tgt[2];
-    float k0 = pk0 - tgt[0], k1 = pk1 - tgt[1], k2 = pk2 - tgt[2];
-    return (k0 * k0 + k1 * k1 + k2 * k2) - (d0 * d0 + d1 * d1 + d2 * d2);
+    return sad_removal_delta(pred[0], pred[1], pred[2], tgt[0], tgt[1], tgt[2], s->col_r, s->col_g, s->col_b, wt);
 }
 
 /* Generic per-pixel gradient driver.  mode 0: loss gradient 2(pred - tgt) with

@@ -1,0 +1,141 @@
+"""ctypes binding of libsad_hip.so, the C ABI declared in include/sad_hip.h.
+
+Every entry point returns an int status; `call` raises SadHipError with the
+library's message when it is non-zero.  The library must be built (see
+__graft_entry__.build or `make -C sad_amd/csrc`); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsad_hip.so")
+
+P = C.c_void_p
+U32 = C.c_uint32
+I32 = C.c_int
+F32 = C.c_float
+
+
+class SadHipError(RuntimeError):
+    pass
+
+
+class FitOptions(C.Structure):
+    """SadhFitOptions (include/sad_hip.h)."""
+    _fields_ = [
+        ("iterations", I32), ("target_bpp", F32), ("init_mode", I32), ("n_sites", I32), ("max_sites", I32),
+        ("prune_percentile", F32), ("prune_start", I32), ("prune_freq", I32), ("prune_end", I32),
+        ("prune_during_densify", I32), ("densify_enabled", I32), ("densify_start", I32), ("densify_freq", I32),
+        ("densify_end", I32), ("densify_percentile", F32), ("densify_score_alpha", F32),
+        ("cand_update_freq", I32), ("cand_update_passes", I32), ("cand_downscale", I32),
+        ("cand_radius_scale", F32), ("cand_radius_probes", U32), ("cand_inject_count", U32),
+        ("cand_hilbert_window", U32), ("cand_hilbert_probes", U32),
+        ("lr_scale", F32), ("lr_pos", F32), ("lr_tau", F32), ("lr_radius", F32), ("lr_color", F32),
+        ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32),
+        ("log_freq", I32), ("profile_gradients", I32),
+    ]
+
+
+class FitResult(C.Structure):
+    """SadhFitResult (include/sad_hip.h)."""
+    _fields_ = [
+        ("final_psnr", F32), ("best_psnr", F32), ("actual_sites", I32), ("active_estimate", I32),
+        ("active_count", I32), ("densify_percentile", F32), ("prune_percentile", F32), ("capacity", I32),
+        ("n_logs", I32), ("n_densify", I32), ("n_prune", I32), ("n_cand_updates", I32),
+        ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
+        ("grad_bytes", C.c_double), ("grad_ms_trace", C.POINTER(F32)), ("grad_ms_trace_cap", I32),
+    ]
+
+
+# name -> argument types (all return int)
+SIGNATURES = {
+    "sadh_render_voronoi": [P, P, P, P, F32, U32, I32, I32, I32, I32, P],
+    "sadh_render_backward": [P, P, P, P, P, F32, U32, I32, I32, I32, I32, P],
+    "sadh_init_candidates": [P, P, U32, U32, I32, I32, I32, P],
+    "sadh_clear_candidates": [P, P, I32, I32, P],
+    "sadh_jfa_seed": [P, P, U32, I32, I32, I32, P],
+    "sadh_jfa_flood": [P, P, P, U32, U32, F32, I32, I32, I32, I32, I32, P],
+    "sadh_pack_candidate_sites": [P, P, U32, P],
+    "sadh_update_candidates": [P, P, P, P, P, U32, U32, F32, U32, F32, U32, U32, P, P, U32, U32,
+                               I32, I32, I32, I32, I32, P],
+    "sadh_hilbert_pairs": [P, P, U32, U32, I32, I32, I32, P],
+    "sadh_hilbert_order": [P, P, P, U32, P],
+    "sadh_compute_gradients_tiled": [P, P, P, P, P] + [P] * 10 + [P, F32, U32, P, U32, F32, I32, I32, I32, I32, P],
+    "sadh_tau_diffuse": [P, P, P, P, P, P, U32, F32, I32, I32, I32, P],
+    "sadh_adam_update": [P, P] + [P] * 10 + [F32] * 9 + [U32, U32, I32, I32, P],
+    "sadh_adam_update_clear": [P, P] + [P] * 10 + [F32] * 9 + [U32, U32, I32, I32, P],
+    "sadh_clear_gradients": [P] * 10 + [U32, P],
+    "sadh_clear_buffer": [P, U32, P],
+    "sadh_compute_site_stats_simple": [P, P, P, P, P, F32, U32] + [P] * 8 + [I32, I32, I32, I32, P],
+    "sadh_compute_densify_score_pairs": [P, P, P, P, U32, F32, F32, U32, P],
+    "sadh_compute_prune_score_pairs": [P, P, P, U32, F32, U32, P],
+    "sadh_radix_sort_pairs": [P, P, P, P, U32, U32, P],
+    "sadh_write_split_indices": [P, P, U32, P],
+    "sadh_split_sites": [P, P, P, U32] + [P] * 7 + [U32, P, I32, I32, P],
+    "sadh_prune_sites_by_index": [P, P, U32, P],
+    "sadh_compute_psnr": [P, P, P, P, I32, I32, P],
+    "sadh_count_active_sites": [P, P, U32, P],
+    "sadh_fit_plan": [C.POINTER(FitOptions), I32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32),
+                      C.POINTER(I32)],
+    "sadh_fit": [C.POINTER(FitOptions), P, P, I32, I32, P, I32, P, I32, C.POINTER(FitResult)],
+    "sadh_render_sites": [P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
+                          C.POINTER(F32)],
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libsad_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SadHipError(f"{LIB_PATH} is missing: build it with `make -C sad_amd/csrc` "
+                          "or __graft_entry__.build()")
+    try:
+        # torch ships its own HIP runtime; importing it first makes the library
+        # bind to that same runtime, so torch tensors and streams can be passed in.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.sadh_last_error.restype = C.c_char_p
+    lib.sadh_last_error.argtypes = []
+    lib.sadh_version.restype = C.c_char_p
+    lib.sadh_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> None:
+    """Call entry point `name`; raise SadHipError on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.sadh_last_error().decode(errors="replace")
+        raise SadHipError(f"{name} failed with status {rc}: {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device/host address of a torch tensor or numpy array (None for None)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/sad_hip.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "sad_hip.h")
+    with open(hdr) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(sadh_[a-z0-9_]+)\s*\(", text)))

@@ -1,29 +1,41 @@
 /*
- * k_pixel.hip - per-pixel kernels: softmax render forward, tiled hashed gradient
+ * k_pixel.hip - per-pixel kernels: softmax render forward, tiled gradient
  * backward (forward recomputed), autograd backward into [N][10], and the
  * densify site statistics.
  *
- * Design (gfx950):
- *   - One 256-thread workgroup per 16x16-pixel tile.  Tiles are mapped to
- *     workgroups so that consecutive tiles share an XCD (and its L2).
- *   - Stage 1: every (pixel, candidate) pair inserts its site id into an LDS
- *     hash of H slots (multiplicative hash, linear probing, CAS only on empty
- *     slots).  Stage 2: one thread per occupied slot loads the 40-byte site once
- *     and stores the per-site terms of the forward (tau, e^a, e^-a, r/S, colour)
- *     in LDS, so the exp/divisions run once per site per tile instead of once
- *     per pixel-candidate.  Ids that do not fit (H slots, 8 probes) are evaluated
- *     from global memory and accumulate with global atomics.
- *   - Stage 3: a wave64 covers 8 pixels x 8 candidates: lane = 8 * pixel +
- *     candidate.  The softmax max/sum and the colour blend are 8-lane DPP
- *     reductions, and each lane then owns one candidate's gradient.  Because a
- *     wave's atomics now target the candidates of 8 pixels (not the same
- *     candidate rank of 64 neighbouring pixels, which is mostly one site), LDS
- *     atomic conflicts drop roughly 8x compared with thread-per-pixel.
- *   - Stage 4: occupied slots are flushed to the SoA gradient arrays with one
- *     global float atomic per parameter.
- * Numerics: per-site terms use the exact primitives; the per-pixel distance and
- * weights use the hardware sqrt/exp (~1 ulp).  Parity with the oracle is a
- * tolerance for these kernels.
+ * One 256-thread workgroup per 16x16-pixel tile; consecutive tiles are mapped
+ * onto one XCD so neighbouring tiles, which share most candidate sites, share
+ * an L2.  Stages:
+ *   1. each thread (one pixel) loads its 8 candidate ids and inserts them into
+ *      an LDS hash of H slots (multiplicative hash, linear probing, CAS only on
+ *      an empty slot).  Ids that find no slot within 8 probes are "overflow".
+ *   2. one thread per occupied slot loads the 40-byte site once and stores the
+ *      per-site terms of the forward (tau, e^a, 1/e^a, r/S, colour) in LDS; the
+ *      occupied slots are compacted into a dense list.
+ *   3. each thread computes its pixel's softmax over the 8 candidates from the
+ *      staged terms.  The render writes the colour here.  The backward kernels
+ *      store the weights and the pixel's (pred, dL/dpred) in LDS and take a rank
+ *      per (slot, pixel) with an integer LDS atomic; overflow candidates are
+ *      differentiated right here and go to global memory with float atomics.
+ *   4. a scan of the per-slot counts turns ranks into positions of a per-slot
+ *      entry list (pixel, candidate rank).
+ *   5. site-centric accumulation: 8 lanes per slot walk the slot's entries,
+ *      differentiate each (pixel, candidate) pair and sum in registers; an
+ *      8-lane DPP reduction then gives one global float atomic per
+ *      (site, parameter) and tile.
+ * Why not LDS float atomics (the reference's tile hash, gradients.cu:422-459):
+ * on gfx950, ds_add_f32 measured ~190 cycles per wave instruction per CU
+ * against ~5 for ds_add_u32 and ~8 for a read-add-write, and the hashed
+ * accumulation needs 10 of them per pixel-candidate; here only one integer
+ * atomic per pixel-candidate remains.
+ * Numerics: the forward (per-site terms, logits, softmax weights, blended
+ * colour) uses the exact primitives of sad_math.h in the oracle's operation
+ * order, so weights and colours are bit-identical to the CPU restatement.  This
+ * matters beyond the render: dL/dlogit = w g.(c - pred) cancels when one
+ * candidate dominates a pixel, so ulp-level differences in pred would become
+ * large relative differences in the gradient.  The per-pair derivative is
+ * sad_grad_terms, the function the oracle evaluates, so GPU and CPU differ only
+ * in the order of the fp32 sums.
  */
 #include "sad_device.h"
 
@@ -33,7 +45,7 @@ enum PixelMode { MODE_RENDER = 0, MODE_GRAD = 1, MODE_BWD = 2, MODE_STATS = 3 };
 
 constexpr uint16_t SLOT_INVALID = 0xFFFFu;   /* id >= site count or pixel not processed */
 constexpr uint16_t SLOT_OVERFLOW = 0xFFFEu;  /* valid id without a hash slot */
-constexpr int NACC = 10;
+constexpr uint16_t ENTRY_FIRST = 0x8000u;    /* entry flag: first rank of its site in the pixel */
 
 template <int H>
 struct TileLds {
@@ -41,9 +53,16 @@ struct TileLds {
     float4 p0[H];              /* pos.x, pos.y, tau, r/S      (pos.x < 0: inactive) */
     float4 p1[H];              /* dir.x, dir.y, e^a, 1/e^a */
     float4 p2[H];              /* r, g, b, 1/max(e^a, 1e-8) */
-    float acc[NACC][H];
-    float delta[H];
-    uint16_t slot[256 * 8];
+    uint32_t count[H];         /* entries per slot */
+    uint32_t start[H];         /* first entry of each slot in `list` */
+    uint16_t occ[H];           /* dense list of occupied slots */
+    uint32_t n_occ;
+    uint16_t slot[256][8];     /* per pixel: slot of each candidate rank (INVALID if unused) */
+    float w[256][8];           /* per pixel: softmax weights */
+    float4 px[256];            /* per pixel: pred.rgb, err (STATS) */
+    float4 pg[256];            /* per pixel: dL/dpred.rgb */
+    float4 pt[256];            /* per pixel: target.rgb (removal deltas) */
+    uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST) */
 };
 
 struct SiteTerms {
@@ -80,6 +99,29 @@ __device__ __forceinline__ uint16_t hash_insert(TileLds<H>& L, uint32_t id)
     return SLOT_OVERFLOW;
 }
 
+/* normalised, clamped d^T G d of the det-1 metric for a given 1/e^a */
+__device__ __forceinline__ float d2_safe(const SiteTerms& t, float dx, float dy, float l2, float iss,
+                                         float* proj_out, float* perp2_out)
+{
+    float diff2 = fmaf(dx, dx, dy * dy);
+    float proj = fmaf(t.p1.x, dx, t.p1.y * dy);
+    float proj2 = proj * proj;
+    float perp2 = fmaxf(diff2 - proj2, 0.0f);
+    *proj_out = proj;
+    *perp2_out = perp2;
+    return fmaxf(fmaf(t.p1.z, proj2, l2 * perp2) * iss, 1e-8f);
+}
+
+/* Derivative through one (pixel, candidate) pair: sad_grad_terms of sad_math.h
+ * (gradients.cu:352-420) on the staged site terms. */
+__device__ __forceinline__ void candidate_grad(const SiteTerms& t, float ux, float uy, float w, float gr, float gg,
+                                               float gb, float pr, float pg_, float pb, float iss, float inv_scale,
+                                               float c[10])
+{
+    sad_grad_terms(t.p0.x, t.p0.y, t.p0.z, t.p0.w, t.p1.x, t.p1.y, t.p1.z, t.p2.w, t.p2.x, t.p2.y, t.p2.z, ux, uy,
+                   w, gr, gg, gb, pr, pg_, pb, iss, inv_scale, c);
+}
+
 struct PixelArgs {
     const uint32_t* cand0;
     const uint32_t* cand1;
@@ -88,7 +130,7 @@ struct PixelArgs {
     const float* mask;         /* GRAD, STATS */
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3]; BWD: [N][10] */
-    float* g[NACC];            /* GRAD: 10 SoA arrays; STATS: 8 arrays */
+    float* g[10];              /* GRAD: 10 SoA arrays; STATS: 8 arrays */
     float* removal;            /* GRAD with removal */
     float iss, inv_scale;
     uint32_t n;
@@ -101,41 +143,40 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
     __shared__ TileLds<H> L;
     const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t k = tid & 7u;
     const uint32_t tile = xcd_tile(blockIdx.x, A.ntiles);
-    const int tx0 = (int)(tile % A.tiles_x) * 16;
-    const int ty0 = (int)(tile / A.tiles_x) * 16;
-    const uint32_t* cplane = (k < 4u) ? A.cand0 : A.cand1;
-    const uint32_t kk = k & 3u;
+    const int tx0 = (int)(tile % A.tiles_x) * 16, ty0 = (int)(tile / A.tiles_x) * 16;
+    const int x = tx0 + (int)(tid & 15u);
+    const int y = ty0 + (int)(tid >> 4);
+    const bool inb = x < A.width && y < A.height;
+    const uint32_t pix = inb ? (uint32_t)(y * A.width + x) : 0u;
+    const float ux = (float)x, uy = (float)y;
 
     for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
         L.key[s] = SAD_EMPTY;
-        if (MODE != MODE_RENDER) {
-#pragma unroll
-            for (int j = 0; j < NACC; ++j) L.acc[j][s] = 0.0f;
-            L.delta[s] = 0.0f;
-        }
+        L.count[s] = 0u;
     }
     __syncthreads();
 
-    /* stage 1: hash the tile's candidate ids */
-    for (int r = 0; r < 8; ++r) {
-        uint32_t p = (uint32_t)r * 32u + (tid >> 3);
-        int x = tx0 + (int)(p & 15u), y = ty0 + (int)(p >> 4);
-        uint16_t slot = SLOT_INVALID;
-        bool act = x < A.width && y < A.height;
-        if (act && (MODE == MODE_GRAD || MODE == MODE_STATS)) act = A.mask[y * A.width + x] > 0.0f;
+    /* stage 1: this pixel's candidate ids into the tile hash */
+    bool act = inb;
+    if (act && (MODE == MODE_GRAD || MODE == MODE_STATS)) act = A.mask[pix] > 0.0f;
+    uint32_t ids[8];
+    uint16_t slots[8];
+    {
+        uint4 a = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY), b = a;
         if (act) {
             int cell = sad_cand_index(x, y, A.width, A.height, A.cw, A.ch);
-            uint32_t id = cplane[4u * (uint32_t)cell + kk];
-            if (id < A.n) slot = hash_insert<H>(L, id);
+            a = reinterpret_cast<const uint4*>(A.cand0)[cell];
+            b = reinterpret_cast<const uint4*>(A.cand1)[cell];
         }
-        L.slot[p * 8u + k] = slot;
+        ids[0] = a.x; ids[1] = a.y; ids[2] = a.z; ids[3] = a.w;
+        ids[4] = b.x; ids[5] = b.y; ids[6] = b.z; ids[7] = b.w;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) slots[k] = (act && ids[k] < A.n) ? hash_insert<H>(L, ids[k]) : SLOT_INVALID;
     }
     __syncthreads();
 
-    /* stage 2: per-site terms of the occupied slots */
+    /* stage 2: per-site terms of the occupied slots; dense list of them */
     for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
         uint32_t id = L.key[s];
         if (id == SAD_EMPTY) continue;
@@ -144,188 +185,244 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
         L.p1[s] = t.p1;
         L.p2[s] = t.p2;
     }
+    if (MODE != MODE_RENDER && tid < 64u) {
+        uint32_t base = 0u;
+        const unsigned long long lt = (tid == 0u) ? 0ull : (~0ull >> (64u - tid));
+        for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
+            bool occ = L.key[s0 + tid] != SAD_EMPTY;
+            unsigned long long b = __ballot(occ);
+            if (occ) L.occ[base + (uint32_t)__popcll(b & lt)] = (uint16_t)(s0 + tid);
+            base += (uint32_t)__popcll(b);
+        }
+        if (tid == 0u) L.n_occ = base;
+    }
     __syncthreads();
 
-    /* stage 3: 8 lanes per pixel */
-    for (int r = 0; r < 8; ++r) {
-        uint32_t p = (uint32_t)r * 32u + (tid >> 3);
-        int x = tx0 + (int)(p & 15u), y = ty0 + (int)(p >> 4);
-        bool inb = x < A.width && y < A.height;
-        uint16_t slot = L.slot[p * 8u + k];
-        uint32_t pix = inb ? (uint32_t)(y * A.width + x) : 0u;
+    /* stage 3: softmax of this pixel (render.cu:34-91) */
+    float logit[8];
+    bool valid[8];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint16_t s = slots[k];
         SiteTerms t;
         t.p0 = make_float4(-1.0f, -1.0f, 0.0f, 0.0f);
-        t.p1 = t.p2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t id = SAD_EMPTY;
-        if (slot < (uint16_t)H) {
-            t.p0 = L.p0[slot];
-            t.p1 = L.p1[slot];
-            t.p2 = L.p2[slot];
-            id = L.key[slot];
-        } else if (slot == SLOT_OVERFLOW) {
-            int cell = sad_cand_index(x, y, A.width, A.height, A.cw, A.ch);
-            id = cplane[4u * (uint32_t)cell + kk];
-            t = site_terms(A.sites, id, A.inv_scale);
+        t.p1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (s < (uint16_t)H) {
+            t.p0 = L.p0[s];
+            t.p1 = L.p1[s];
+        } else if (s == SLOT_OVERFLOW) {
+            SiteTerms f = site_terms(A.sites, ids[k], A.inv_scale);
+            t.p0 = f.p0;
+            t.p1 = f.p1;
         }
-        const bool valid = slot != SLOT_INVALID && t.p0.x >= 0.0f;
-        const float ux = (float)x, uy = (float)y;
-
-        /* forward: logit = -tau * dmix (render.cu:52-55) */
-        float dx = ux - t.p0.x, dy = uy - t.p0.y;
-        float diff2 = fmaf(dx, dx, dy * dy);
-        float proj = fmaf(t.p1.x, dx, t.p1.y * dy);
-        float proj2 = proj * proj;
-        float perp2 = fmaxf(diff2 - proj2, 0.0f);
-        float d2s_f = fmaxf(fmaf(t.p1.z, proj2, t.p1.w * perp2) * A.iss, 1e-8f);
-        float logit = valid ? -(t.p0.z * (fast_sqrt(d2s_f) - t.p0.w)) : -INFINITY;
-        float mx = group8_max(logit);
-        bool has = mx > -INFINITY;
-        /* the autograd kernel also skips a +inf maximum (sad_bindings.cu:72-74) */
-        if (MODE == MODE_BWD) has = has && mx < INFINITY;
-        float e = (valid && has) ? fast_exp(logit - mx) : 0.0f;
-        float sum = group8_sum(e);
-        float w = e * (1.0f / fmaxf(sum, 1e-8f));
-        float wc = w;
-        if (MODE == MODE_RENDER) wc = (valid && !isnan(w) && !isinf(w)) ? w : 0.0f;
-        else wc = valid ? w : 0.0f;
-        float pr = group8_sum(wc * t.p2.x);
-        float pg = group8_sum(wc * t.p2.y);
-        float pb = group8_sum(wc * t.p2.z);
-
-        if (MODE == MODE_RENDER) {
-            if (inb && k == 0u) {
-                float* o = A.out + 3u * pix;
-                o[0] = has ? pr : 0.0f;
-                o[1] = has ? pg : 0.0f;
-                o[2] = has ? pb : 0.0f;
-            }
-            continue;
-        }
-
-        float tr = 0.0f, tg = 0.0f, tb = 0.0f;
-        if ((MODE == MODE_GRAD || MODE == MODE_STATS) && inb) {
-            const float* tp = A.target + 3u * pix;
-            tr = tp[0]; tg = tp[1]; tb = tp[2];
-        }
-
-        if (MODE == MODE_STATS || (MODE == MODE_GRAD && REMOVAL)) {
-            /* per unique site of the pixel: summed weight, counted at its first rank
-             * (utils.cu:529-569, gradients.cu:295-321) */
-            uint32_t myid = valid ? id : (0xFFFFFF00u | lane);
-            float wt = 0.0f;
-            bool first = true;
-            const int base = (int)(lane & ~7u);
+        valid[k] = s != SLOT_INVALID && t.p0.x >= 0.0f;
+        float proj, perp2;
+        float d2s = d2_safe(t, ux - t.p0.x, uy - t.p0.y, t.p1.w, A.iss, &proj, &perp2);
+        logit[k] = valid[k] ? -(t.p0.z * (sqrtf(d2s) - t.p0.w)) : -INFINITY;
+        mx = fmaxf(mx, logit[k]);
+    }
+    bool has = mx > -INFINITY;
+    if (MODE == MODE_BWD) has = has && mx < INFINITY;   /* sad_bindings.cu:72-74 */
+    float w[8];
+    float sum = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                uint32_t oj = (uint32_t)__shfl((int)myid, base + j);
-                float wj = __shfl(w, base + j);
-                if (oj == myid) {
-                    wt += wj;
-                    if (j < (int)k) first = false;
-                }
-            }
-            if (valid && has && first && wt > 0.0f) {
-                if (MODE == MODE_STATS) {
-                    float er = pr - tr, eg = pg - tg, eb = pb - tb;
-                    float err = er * er + eg * eg + eb * eb;
-                    float we = wt * fminf(err, 1.0f);
-                    float v[8] = {wt, wt * err, we, we * ux, we * uy, we * ux * ux, we * ux * uy, we * uy * uy};
-                    if (slot < (uint16_t)H) {
+    for (int k = 0; k < 8; ++k) {
+        w[k] = (valid[k] && has) ? sad_expf(logit[k] - mx) : 0.0f;
+        sum += w[k];
+    }
+    const float inv = 1.0f / fmaxf(sum, 1e-8f);
+    float pr = 0.0f, pgr = 0.0f, pb = 0.0f;
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) atomicAdd(&L.acc[j][slot], v[j]);
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) unsafeAtomicAdd(A.g[j] + id, v[j]);
-                    }
-                } else {
-                    /* removalDeltaForSite, gradients.cu:82-100 */
-                    float keep = 1.0f - wt;
-                    float dl;
-                    if (keep < 1e-4f) {
-                        dl = 1.0e6f;
-                    } else {
-                        float ik = 1.0f / keep;
-                        float k0 = (pr - wt * t.p2.x) * ik - tr;
-                        float k1 = (pg - wt * t.p2.y) * ik - tg;
-                        float k2 = (pb - wt * t.p2.z) * ik - tb;
-                        float d0 = pr - tr, d1 = pg - tg, d2 = pb - tb;
-                        dl = (k0 * k0 + k1 * k1 + k2 * k2) - (d0 * d0 + d1 * d1 + d2 * d2);
-                    }
-                    if (slot < (uint16_t)H) atomicAdd(&L.delta[slot], dl);
-                    else unsafeAtomicAdd(A.removal + id, dl);
-                }
-            }
-        }
-        if (MODE == MODE_STATS) continue;
-
-        /* backward of one candidate (gradients.cu:352-420) */
-        if (!(valid && has)) continue;
-        float gx_, gy_, gz_;
-        if (MODE == MODE_GRAD) {
-            gx_ = 2.0f * (pr - tr); gy_ = 2.0f * (pg - tg); gz_ = 2.0f * (pb - tb);
-        } else {
-            const float* go = A.grad_out + 3u * pix;
-            gx_ = go[0]; gy_ = go[1]; gz_ = go[2];
-        }
-        float c[NACC];
-        c[4] = w * gx_;
-        c[5] = w * gy_;
-        c[6] = w * gz_;
-        float dlogit = c[4] * (t.p2.x - pr) + c[5] * (t.p2.y - pg) + c[6] * (t.p2.z - pb);
-        float tau = t.p0.z;
-        float l1 = t.p1.z, l2 = t.p2.w;
-        float d2s = fmaxf(fmaf(l1, proj2, l2 * perp2) * A.iss, 1e-8f);
-        float isd = fast_rsq(d2s);
-        float dmix = fast_sqrt(d2s) - t.p0.w;
-        float dl12 = l1 - l2;
-        float gxd = fmaf(dl12 * t.p1.x, proj, l2 * dx);
-        float gyd = fmaf(dl12 * t.p1.y, proj, l2 * dy);
-        float cpos = dlogit * (tau * A.iss * isd);
-        c[0] = cpos * gxd;
-        c[1] = cpos * gyd;
-        c[2] = -dlogit * dmix * tau;
-        c[3] = dlogit * tau * A.inv_scale;
-        float hq = -dlogit * tau * 0.5f * isd;
-        c[9] = hq * (fmaf(l1, proj2, -(l2 * perp2)) * A.iss);
-        float cd = 2.0f * dl12 * proj * A.iss;
-        c[7] = hq * (cd * dx);
-        c[8] = hq * (cd * dy);
-        if (slot < (uint16_t)H) {
-#pragma unroll
-            for (int j = 0; j < NACC; ++j) atomicAdd(&L.acc[j][slot], c[j]);
-        } else if (MODE == MODE_GRAD) {
-#pragma unroll
-            for (int j = 0; j < NACC; ++j) unsafeAtomicAdd(A.g[j] + id, c[j]);
-        } else {
-#pragma unroll
-            for (int j = 0; j < NACC; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, c[j]);
+    for (int k = 0; k < 8; ++k) {
+        w[k] = w[k] * inv;
+        float wc = w[k];
+        if (MODE == MODE_RENDER && (isnan(wc) || isinf(wc))) wc = 0.0f;
+        if (valid[k]) {
+            float4 c = (slots[k] < (uint16_t)H) ? L.p2[slots[k]] : site_terms(A.sites, ids[k], A.inv_scale).p2;
+            pr += wc * c.x;
+            pgr += wc * c.y;
+            pb += wc * c.z;
         }
     }
 
-    if (MODE == MODE_RENDER) return;
+    if (MODE == MODE_RENDER) {
+        if (inb) {
+            float* o = A.out + 3u * pix;
+            o[0] = has ? pr : 0.0f;
+            o[1] = has ? pgr : 0.0f;
+            o[2] = has ? pb : 0.0f;
+        }
+        return;
+    }
+
+    /* dL/dpred and the per-pixel data the site-centric pass reads */
+    const bool contrib = act && has;
+    float gr = 0.0f, gg = 0.0f, gb = 0.0f, err = 0.0f;
+    float tr = 0.0f, tg = 0.0f, tb = 0.0f;
+    if (contrib) {
+        if (MODE == MODE_BWD) {
+            const float* go = A.grad_out + 3u * pix;
+            gr = go[0]; gg = go[1]; gb = go[2];
+        } else {
+            const float* tp = A.target + 3u * pix;
+            tr = tp[0]; tg = tp[1]; tb = tp[2];
+            float dr = pr - tr, dg = pgr - tg, db = pb - tb;
+            gr = 2.0f * dr; gg = 2.0f * dg; gb = 2.0f * db;
+            err = dr * dr + dg * dg + db * db;
+        }
+    }
+    L.px[tid] = make_float4(pr, pgr, pb, err);
+    L.pg[tid] = make_float4(gr, gg, gb, 0.0f);
+    if (REMOVAL) L.pt[tid] = make_float4(tr, tg, tb, 0.0f);
+    uint32_t rank[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        L.w[tid][k] = w[k];
+        const uint16_t s = slots[k];
+        const bool use = contrib && valid[k];
+        rank[k] = 0u;
+        if (use && s < (uint16_t)H) rank[k] = atomicAdd(&L.count[s], 1u);
+        L.slot[tid][k] = (use && s < (uint16_t)H) ? s : SLOT_INVALID;
+        if (use && s == SLOT_OVERFLOW) {
+            /* an id without a hash slot: finish its contribution here, in global memory */
+            SiteTerms t = site_terms(A.sites, ids[k], A.inv_scale);
+            bool first = true;
+            float wt = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (ids[j] == ids[k] && valid[j]) {
+                    wt += w[j];
+                    if (j < k) first = false;
+                }
+            }
+            const uint32_t id = ids[k];
+            if (MODE == MODE_STATS) {
+                if (first && wt > 0.0f) {
+                    float we = wt * fminf(err, 1.0f);
+                    float v[8] = {wt, wt * err, we, we * ux, we * uy, we * ux * ux, we * ux * uy, we * uy * uy};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) unsafeAtomicAdd(A.g[j] + id, v[j]);
+                }
+            } else {
+                float c[10];
+                candidate_grad(t, ux, uy, w[k], gr, gg, gb, pr, pgr, pb, A.iss, A.inv_scale, c);
+                if (MODE == MODE_GRAD) {
+#pragma unroll
+                    for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.g[j] + id, c[j]);
+                    if (REMOVAL && first && wt > 0.0f)
+                        unsafeAtomicAdd(A.removal + id,
+                                        sad_removal_delta(pr, pgr, pb, tr, tg, tb, t.p2.x, t.p2.y, t.p2.z, wt));
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, c[j]);
+                }
+            }
+        }
+    }
     __syncthreads();
 
-    /* stage 4: flush the tile's sums */
-    for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
-        uint32_t id = L.key[s];
-        if (id == SAD_EMPTY || !(L.p0[s].x >= 0.0f)) continue;
-        if (MODE == MODE_GRAD) {
+    /* stage 4: list offsets = exclusive scan of the per-slot counts (wave 0) */
+    if (tid < 64u) {
+        uint32_t run = 0u;
+        for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
+            uint32_t c = L.count[s0 + tid];
+            uint32_t v = c;
 #pragma unroll
-            for (int j = 0; j < NACC; ++j) unsafeAtomicAdd(A.g[j] + id, L.acc[j][s]);
-            if (REMOVAL && L.delta[s] != 0.0f) unsafeAtomicAdd(A.removal + id, L.delta[s]);
-        } else if (MODE == MODE_BWD) {
+            for (uint32_t off = 1; off < 64u; off <<= 1) {
+                uint32_t o = (uint32_t)__shfl_up((int)v, off);
+                if (tid >= off) v += o;
+            }
+            L.start[s0 + tid] = run + v - c;
+            run += (uint32_t)__shfl((int)v, 63);
+        }
+    }
+    __syncthreads();
 #pragma unroll
-            for (int j = 0; j < NACC; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, L.acc[j][s]);
-        } else {
+    for (int k = 0; k < 8; ++k) {
+        const uint16_t s = L.slot[tid][k];
+        if (s != SLOT_INVALID) {
+            bool first = true;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) unsafeAtomicAdd(A.g[j] + id, L.acc[j][s]);
+            for (int j = 0; j < k; ++j) first = first && (L.slot[tid][j] != s);
+            L.list[L.start[s] + rank[k]] = (uint16_t)((tid << 3) | (uint32_t)k | (first ? ENTRY_FIRST : 0u));
+        }
+    }
+    __syncthreads();
+
+    /* stage 5: 8 lanes per occupied slot walk its entry list */
+    const uint32_t n_occ = L.n_occ;
+    const uint32_t part = tid & 7u;
+    for (uint32_t base = 0; base < n_occ; base += 32u) {
+        const uint32_t u = base + (tid >> 3);
+        const bool live = u < n_occ;
+        const uint32_t s = live ? (uint32_t)L.occ[u] : 0u;
+        const uint32_t beg = live ? L.start[s] : 0u;
+        const uint32_t end = live ? beg + L.count[s] : 0u;
+        SiteTerms t;
+        t.p0 = L.p0[s];
+        t.p1 = L.p1[s];
+        t.p2 = L.p2[s];
+        float acc[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) acc[j] = 0.0f;
+        float dsum = 0.0f;
+        for (uint32_t e = beg + part; e < end; e += 8u) {
+            const uint32_t ent = L.list[e];
+            const uint32_t p = (ent & 0x7FFFu) >> 3, k = ent & 7u;
+            const float ex = (float)(tx0 + (int)(p & 15u)), ey = (float)(ty0 + (int)(p >> 4));
+            const float4 pp = L.px[p];
+            const float4 gq = L.pg[p];
+            if (MODE == MODE_STATS || (MODE == MODE_GRAD && REMOVAL)) {
+                if (ent & ENTRY_FIRST) {
+                    float wt = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) wt += (L.slot[p][j] == (uint16_t)s) ? L.w[p][j] : 0.0f;
+                    if (wt > 0.0f) {
+                        if (MODE == MODE_STATS) {
+                            const float er = pp.w;
+                            const float we = wt * fminf(er, 1.0f);
+                            acc[0] += wt; acc[1] += wt * er; acc[2] += we;
+                            acc[3] += we * ex; acc[4] += we * ey;
+                            acc[5] += we * ex * ex; acc[6] += we * ex * ey; acc[7] += we * ey * ey;
+                        } else {
+                            const float4 tq = L.pt[p];
+                            dsum += sad_removal_delta(pp.x, pp.y, pp.z, tq.x, tq.y, tq.z, t.p2.x, t.p2.y, t.p2.z, wt);
+                        }
+                    }
+                }
+            }
+            if (MODE != MODE_STATS) {
+                float c[10];
+                candidate_grad(t, ex, ey, L.w[p][k], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss, A.inv_scale, c);
+#pragma unroll
+                for (int j = 0; j < 10; ++j) acc[j] += c[j];
+            }
+        }
+        constexpr int NOUT = (MODE == MODE_STATS) ? 8 : 10;
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) acc[j] = group8_sum(acc[j]);
+        if (MODE == MODE_GRAD && REMOVAL) dsum = group8_sum(dsum);
+        if (live && part == 0u && t.p0.x >= 0.0f) {
+            const uint32_t id = L.key[s];
+            if (MODE == MODE_BWD) {
+#pragma unroll
+                for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, acc[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < NOUT; ++j) unsafeAtomicAdd(A.g[j] + id, acc[j]);
+                if (MODE == MODE_GRAD && REMOVAL && dsum != 0.0f) unsafeAtomicAdd(A.removal + id, dsum);
+            }
         }
     }
 }
 
 /* ------------------------------------------------------------ launchers -- */
 
-constexpr int HASH_SLOTS = 256;
+#ifndef SAD_HASH_SLOTS
+#define SAD_HASH_SLOTS 128
+#endif
+constexpr int HASH_SLOTS = SAD_HASH_SLOTS;
 
 template <int MODE, bool REMOVAL>
 static int launch_pixel(PixelArgs& A, hipStream_t stream, const char* what)
@@ -369,6 +466,8 @@ int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float*
     A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     return launch_pixel<MODE_STATS, false>(A, stream, "compute_site_stats");
 }
+
+int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream);
 
 }  // namespace sadh
 
@@ -419,6 +518,22 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, c
                           grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
     return launch_gradients(cand0, cand1, target, mask, g, sites, inv_scale_sq, site_count, removal_delta,
                             compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream));
+}
+
+int sadh_compute_site_stats_simple(const uint32_t* cand0, const uint32_t* cand1, const float* target,
+                                   const float* mask, const SadSite* sites, float inv_scale_sq, uint32_t site_count,
+                                   float* mass, float* energy, float* err_w, float* err_wx, float* err_wy,
+                                   float* err_wxx, float* err_wxy, float* err_wyy, int width, int height,
+                                   int cand_width, int cand_height, sadh_stream_t stream)
+{
+    float* const st[8] = {mass, energy, err_w, err_wx, err_wy, err_wxx, err_wxy, err_wyy};
+    for (int j = 0; j < 8; ++j) SADH_REQUIRE(st[j], "compute_site_stats_simple: null stats array");
+    SADH_REQUIRE(cand0 && cand1 && target && mask && sites, "compute_site_stats_simple: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "compute_site_stats_simple: bad sizes");
+    int rc = launch_clear(st, 8, site_count, to_hip(stream));
+    if (rc) return rc;
+    return launch_site_stats(cand0, cand1, target, mask, sites, inv_scale_sq, site_count, st, width, height,
+                             cand_width, cand_height, to_hip(stream));
 }
 
 }  // extern "C"

@@ -1,0 +1,616 @@
+/*
+ * k_sites.hip - per-site kernels: fused Adam, gradient clears, tau-gradient
+ * diffusion, densify/prune scores and site edits, Hilbert keys, metrics.
+ *
+ * All of these stream N-sized arrays once (HBM-bound); one thread per site,
+ * 256-thread blocks.  Sites and Adam state are AoS (40 B / 80 B per site, read
+ * and written whole with 8- and 16-byte accesses); gradients are SoA.
+ */
+#include "sad_device.h"
+
+namespace sadh {
+
+/* ---------------------------------------------------------------- Adam --
+ * adamUpdateKernel, adam.cu:5-180.  b1c = 1/(1-beta1^t), b2c = 1/(1-beta2^t)
+ * arrive precomputed (host powf, once per launch).  CLEAR zeroes the ten
+ * gradient entries after reading them (adam_encoder.metal:117-127). */
+struct AdamArgs {
+    SadSite* sites;
+    SadAdamState* st;
+    float* g[10];
+    float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
+    float beta1, beta2, eps, b1c, b2c, norm, maxx, maxy;
+    uint32_t n;
+};
+
+template <bool CLEAR>
+__global__ __launch_bounds__(256) void k_adam(AdamArgs A)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= A.n) return;
+    float g[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = A.g[j][i] * A.norm;
+    if (CLEAR) {
+#pragma unroll
+        for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
+    }
+    float2* sp = reinterpret_cast<float2*>(A.sites + i);
+    float2 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3], s4 = sp[4];
+    /* s0 = pos, s1 = (log_tau, radius), s2 = (r, g), s3 = (b, dir.x), s4 = (dir.y, log_aniso) */
+    if (s0.x < 0.0f) return;
+    float4* ap = reinterpret_cast<float4*>(A.st + i);
+    float4 a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3], a4 = ap[4];
+    /* a0 = (m_pos.x, m_pos.y, v_pos, m_log_tau), a1 = (v_log_tau, m_radius, v_radius, m_r),
+     * a2 = (m_g, m_b, v_r, v_g), a3 = (v_b, m_dir.x, m_dir.y, v_dir), a4 = (m_la, v_la, pad, pad) */
+    const float b1 = A.beta1, b2 = A.beta2, ob1 = 1.0f - A.beta1, ob2 = 1.0f - A.beta2;
+    const float b1c = A.b1c, b2c = A.b2c, eps = A.eps;
+
+    /* position: one second moment for both axes */
+    a0.x = b1 * a0.x + ob1 * g[0];
+    a0.y = b1 * a0.y + ob1 * g[1];
+    a0.z = b2 * a0.z + ob2 * (g[0] * g[0] + g[1] * g[1]);
+    float idp = 1.0f / (sqrtf(a0.z * b2c) + eps);
+    s0.x = sad_clampf(s0.x - A.lr_pos * (a0.x * b1c) * idp, 0.0f, A.maxx);
+    s0.y = sad_clampf(s0.y - A.lr_pos * (a0.y * b1c) * idp, 0.0f, A.maxy);
+    /* log_tau */
+    a0.w = b1 * a0.w + ob1 * g[2];
+    a1.x = b2 * a1.x + ob2 * (g[2] * g[2]);
+    s1.x = s1.x - A.lr_tau * (a0.w * b1c) / (sqrtf(a1.x * b2c) + eps);
+    /* radius in [1, 512] */
+    a1.y = b1 * a1.y + ob1 * g[3];
+    a1.z = b2 * a1.z + ob2 * (g[3] * g[3]);
+    s1.y = sad_clampf(s1.y - A.lr_radius * (a1.y * b1c) / (sqrtf(a1.z * b2c) + eps), 1.0f, 512.0f);
+    /* colour, per channel, in [0, 1] */
+    a1.w = b1 * a1.w + ob1 * g[4];
+    a2.x = b1 * a2.x + ob1 * g[5];
+    a2.y = b1 * a2.y + ob1 * g[6];
+    a2.z = b2 * a2.z + ob2 * (g[4] * g[4]);
+    a2.w = b2 * a2.w + ob2 * (g[5] * g[5]);
+    a3.x = b2 * a3.x + ob2 * (g[6] * g[6]);
+    s2.x = sad_clampf(s2.x - A.lr_color * (a1.w * b1c) / (sqrtf(a2.z * b2c) + eps), 0.0f, 1.0f);
+    s2.y = sad_clampf(s2.y - A.lr_color * (a2.x * b1c) / (sqrtf(a2.w * b2c) + eps), 0.0f, 1.0f);
+    s3.x = sad_clampf(s3.x - A.lr_color * (a2.y * b1c) / (sqrtf(a3.x * b2c) + eps), 0.0f, 1.0f);
+    /* direction: tangent projection, isotropic Adam, renormalisation */
+    float dxv = s3.y, dyv = s4.x;
+    float gd = g[7] * dxv + g[8] * dyv;
+    float tx = g[7] - dxv * gd, ty = g[8] - dyv * gd;
+    a3.y = b1 * a3.y + ob1 * tx;
+    a3.z = b1 * a3.z + ob1 * ty;
+    a3.w = b2 * a3.w + ob2 * (tx * tx + ty * ty);
+    float idd = 1.0f / (sqrtf(a3.w * b2c) + eps);
+    float nx = dxv - A.lr_dir * (a3.y * b1c) * idd;
+    float ny = dyv - A.lr_dir * (a3.z * b1c) * idd;
+    float len2 = nx * nx + ny * ny;
+    if (len2 < 1e-12f) { nx = 1.0f; ny = 0.0f; }
+    else { float il = 1.0f / sqrtf(len2); nx *= il; ny *= il; }
+    s3.y = nx;
+    s4.x = ny;
+    /* log_aniso in [-4, 4] */
+    a4.x = b1 * a4.x + ob1 * g[9];
+    a4.y = b2 * a4.y + ob2 * (g[9] * g[9]);
+    s4.y = sad_clampf(s4.y - A.lr_aniso * (a4.x * b1c) / (sqrtf(a4.y * b2c) + eps), -4.0f, 4.0f);
+
+    sp[0] = s0; sp[1] = s1; sp[2] = s2; sp[3] = s3; sp[4] = s4;
+    ap[0] = a0; ap[1] = a1; ap[2] = a2; ap[3] = a3; ap[4] = a4;
+}
+
+/* clearGradientsKernel / clearBufferKernel, adam.cu:183-219 */
+struct ClearArgs {
+    float* g[10];
+    int count_arrays;
+    uint32_t n;
+};
+
+__global__ __launch_bounds__(256) void k_clear(ClearArgs A)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= A.n) return;
+    for (int j = 0; j < A.count_arrays; ++j) A.g[j][i] = 0.0f;
+}
+
+/* -------------------------------------------------------- tau diffusion --
+ * tau_diffuse_kernel, tau_diffusion.cu:27-89: one Jacobi step over the unique
+ * active neighbours found in the candidate list of the site's home cell.
+ * raw_copy (nullable) receives grad_raw[i], folding the copy the trainer makes
+ * before the first pass (sad_cuda.cu:2335-2337) into that pass. */
+__global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                     const SadSite* __restrict__ sites,
+                                                     const float* __restrict__ graw, const float* __restrict__ gin,
+                                                     float* __restrict__ gout, float* __restrict__ raw_copy,
+                                                     uint32_t n, float lambda, int width, int height, float inv_ds)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float raw = graw[i];
+    if (raw_copy) raw_copy[i] = raw;
+    const float2* sp = reinterpret_cast<const float2*>(sites + i);
+    float2 pos = sp[0];
+    if (lambda <= 0.0f || pos.x < 0.0f) { gout[i] = gin[i]; return; }
+    int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
+    int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
+    uint4 a = c0[py * width + px], b = c1[py * width + px];
+    uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float sum = 0.0f;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t nb = ids[k];
+        bool ok = nb < n && nb != i;
+#pragma unroll
+        for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
+        if (ok && sites[nb].pos_x >= 0.0f) {
+            sum += gin[nb];
+            cnt += 1u;
+        }
+    }
+    float diag = 1.0f + lambda * (float)cnt;
+    gout[i] = (raw + lambda * sum) / diag;
+}
+
+/* ----------------------------------------------------- densify / prune --- */
+
+/* computeDensifyScorePairsKernel, densify_prune.cu:223-254.  pow(m, alpha) is
+ * the exact exp(alpha * log m) of sad_math.h so keys match the CPU bit for bit. */
+__global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict__ sites, const float* __restrict__ mass,
+                                                       const float* __restrict__ energy, SadPair* __restrict__ pairs,
+                                                       uint32_t n, float min_mass, float alpha, uint32_t count)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    uint2 out = make_uint2(SAD_EMPTY, SAD_EMPTY);
+    if (i < n) {
+        float m = mass[i], e = energy[i];
+        if (sites[i].pos_x >= 0.0f && m > min_mass && isfinite(e)) {
+            float den = sad_powf_pos(fmaxf(m, 1e-8f), alpha);
+            float score = fmaxf(e, 0.0f) / den;
+            out = make_uint2(0xFFFFFFFFu - __float_as_uint(score), i);
+        }
+    }
+    reinterpret_cast<uint2*>(pairs)[i] = out;
+}
+
+/* computePruneScorePairsKernel, densify_prune.cu:257-281 */
+__global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__ sites, const float* __restrict__ rdelta,
+                                                     SadPair* __restrict__ pairs, uint32_t n, float delta_norm,
+                                                     uint32_t count)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    uint2 out = make_uint2(SAD_EMPTY, SAD_EMPTY);
+    if (i < n) {
+        float d = rdelta[i] * delta_norm;
+        if (sites[i].pos_x >= 0.0f && isfinite(d)) out = make_uint2(__float_as_uint(fmaxf(d, 0.0f)), i);
+    }
+    reinterpret_cast<uint2*>(pairs)[i] = out;
+}
+
+/* writeSplitIndicesFromSortedKernel, densify_prune.cu:284-292 */
+__global__ __launch_bounds__(256) void k_write_indices(const SadPair* __restrict__ sorted, uint32_t* __restrict__ idx,
+                                                       uint32_t count)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < count) idx[i] = sorted[i].val;
+}
+
+/* pruneSitesByIndexKernel, densify_prune.cu:295-307 */
+__global__ __launch_bounds__(256) void k_prune_sites(SadSite* __restrict__ sites, const uint32_t* __restrict__ idx,
+                                                     uint32_t count)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    uint32_t s = idx[i];
+    if (s == SAD_EMPTY) return;
+    reinterpret_cast<float2*>(sites + s)[0] = make_float2(-1.0f, -1.0f);
+}
+
+/* splitSitesKernel, densify_prune.cu:86-220, with parents >= cur skipped */
+struct SplitArgs {
+    SadSite* sites;
+    SadAdamState* st;
+    const uint32_t* idx;
+    const float *mass, *ew, *ewx, *ewy, *ewxx, *ewxy, *ewyy;
+    const float* target;
+    uint32_t k, cur;
+    int width, height;
+};
+
+__global__ __launch_bounds__(256) void k_split(SplitArgs A)
+{
+    uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    if (gid >= A.k) return;
+    uint32_t pi = A.idx[gid];
+    if (pi >= A.cur) return;
+    SadSite parent = A.sites[pi];
+    if (parent.pos_x < 0.0f) return;
+    float w = A.ew[pi];
+    float ax = parent.dir_x, ay = parent.dir_y;
+    float cx = parent.pos_x, cy = parent.pos_y;
+    float la = parent.log_aniso * 0.8f;
+    const int W = A.width, Hh = A.height;
+    if (w > 1e-3f) {
+        float mx = A.ewx[pi] / w, my = A.ewy[pi] / w;
+        cx = 0.4f * parent.pos_x + 0.6f * mx;
+        cy = 0.4f * parent.pos_y + 0.6f * my;
+        float exx = fmaxf(A.ewxx[pi] / w - mx * mx, 1e-4f);
+        float exy = A.ewxy[pi] / w - mx * my;
+        float eyy = fmaxf(A.ewyy[pi] / w - my * my, 1e-4f);
+        float th = 0.5f * atan2f(2.0f * exy, exx - eyy);
+        ax = cosf(th);
+        ay = sinf(th);
+        float tr = exx + eyy;
+        float disc = sqrtf(fmaxf(0.0f, 0.25f * (exx - eyy) * (exx - eyy) + exy * exy));
+        float l1 = fmaxf(1e-4f, 0.5f * tr + disc);
+        float l2 = fmaxf(1e-4f, 0.5f * tr - disc);
+        la = sad_clampf(-0.5f * logf(l1 / l2), -2.0f, 2.0f);
+    } else {
+        int ix = sad_clampi((int)parent.pos_x, 0, W - 1);
+        int iy = sad_clampi((int)parent.pos_y, 0, Hh - 1);
+        float lum[3][3];
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                int sx = sad_clampi(ix + dx, 0, W - 1), sy = sad_clampi(iy + dy, 0, Hh - 1);
+                const float* c = A.target + 3u * (uint32_t)(sy * W + sx);
+                lum[dy + 1][dx + 1] = 0.299f * c[0] + 0.587f * c[1] + 0.114f * c[2];
+            }
+        float gx = (-lum[0][0] + lum[0][2] - 2.0f * lum[1][0] + 2.0f * lum[1][2] - lum[2][0] + lum[2][2]) / 4.0f;
+        float gy = (-lum[0][0] - 2.0f * lum[0][1] - lum[0][2] + lum[2][0] + 2.0f * lum[2][1] + lum[2][2]) / 4.0f;
+        float g2 = gx * gx + gy * gy;
+        if (g2 > 1e-8f) {
+            float len = sqrtf(g2);
+            if (len > 1e-8f) { ax = gx / len; ay = gy / len; } else { ax = 1.0f; ay = 0.0f; }
+        }
+    }
+    float m = fmaxf(A.mass[pi], 1.0f);
+    float od = sad_clampf(sqrtf(m) * 0.5f, 1.5f, 48.0f);
+    float ox = ax * od, oy = ay * od;
+    float pax = sad_clampf(cx + ox, 0.0f, (float)(W - 1)), pay = sad_clampf(cy + oy, 0.0f, (float)(Hh - 1));
+    float pbx = sad_clampf(cx - ox, 0.0f, (float)(W - 1)), pby = sad_clampf(cy - oy, 0.0f, (float)(Hh - 1));
+    const float* ca = A.target + 3u * (uint32_t)((int)pay * W + (int)pax);
+    const float* cb = A.target + 3u * (uint32_t)((int)pby * W + (int)pbx);
+    uint32_t child = A.cur + gid;
+    if (child == pi) return;
+    SadSite c0 = parent, c1 = parent;
+    c0.pos_x = pax; c0.pos_y = pay; c1.pos_x = pbx; c1.pos_y = pby;
+    c0.col_r = ca[0]; c0.col_g = ca[1]; c0.col_b = ca[2];
+    c1.col_r = cb[0]; c1.col_g = cb[1]; c1.col_b = cb[2];
+    c0.log_tau = c1.log_tau = parent.log_tau - 0.25f;
+    c0.radius = c1.radius = parent.radius * 0.85f;
+    c0.dir_x = c1.dir_x = ax;
+    c0.dir_y = c1.dir_y = ay;
+    c0.log_aniso = c1.log_aniso = la;
+    A.sites[pi] = c0;
+    A.sites[child] = c1;
+    float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4* sp = reinterpret_cast<float4*>(A.st + pi);
+    float4* sc = reinterpret_cast<float4*>(A.st + child);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { sp[j] = z; sc[j] = z; }
+}
+
+/* ------------------------------------------------------------- Hilbert --- */
+
+/* computeHilbertPairsKernel, sad_cuda.cu:1547-1571 */
+__global__ __launch_bounds__(256) void k_hilbert_pairs(const SadSite* __restrict__ sites, SadPair* __restrict__ pairs,
+                                                       uint32_t n, uint32_t padded, int width, int height, int bits)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= padded) return;
+    uint2 out = make_uint2(SAD_EMPTY, 0u);
+    if (i < n) {
+        float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
+        int px = (int)pos.x, py = (int)pos.y;
+        px = px < 0 ? 0 : (px >= width ? width - 1 : px);
+        py = py < 0 ? 0 : (py >= height ? height - 1 : py);
+        out = make_uint2(sad_hilbert_index((uint32_t)px, (uint32_t)py, bits), i);
+    }
+    reinterpret_cast<uint2*>(pairs)[i] = out;
+}
+
+/* writeHilbertOrderKernel, sad_cuda.cu:1573-1584 */
+__global__ __launch_bounds__(256) void k_hilbert_order(const SadPair* __restrict__ sorted, uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ pos, uint32_t n)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = sorted[i].val;
+    order[i] = s;
+    pos[s] = i;
+}
+
+/* ------------------------------------------------------------- metrics --- */
+
+__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+
+/* computePSNRKernel, utils.cu:5-26: block sum of squared error, one atomic per block */
+__global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const float* __restrict__ t,
+                                             const float* __restrict__ mask, float* __restrict__ acc, uint32_t np)
+{
+    __shared__ float part[4];
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    float v = 0.0f;
+    if (i < np && mask[i] > 0.0f) {
+        float d0 = r[3 * i] - t[3 * i], d1 = r[3 * i + 1] - t[3 * i + 1], d2 = r[3 * i + 2] - t[3 * i + 2];
+        v = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
+}
+
+/* countActiveSitesKernel, utils.cu:90-101 */
+__global__ __launch_bounds__(256) void k_count_active(const SadSite* __restrict__ sites, uint32_t* count, uint32_t n)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    bool a = i < n && sites[i].pos_x >= 0.0f;
+    unsigned long long b = __ballot(a);
+    if ((threadIdx.x & 63u) == 0u && b) atomicAdd(count, (uint32_t)__popcll(b));
+}
+
+/* ------------------------------------------------------------ launchers -- */
+
+int launch_adam(SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos, float lr_tau, float lr_radius,
+                float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps, uint32_t t,
+                uint32_t n, int width, int height, bool clear, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    AdamArgs A;
+    A.sites = sites; A.st = st;
+    for (int j = 0; j < 10; ++j) A.g[j] = g[j];
+    A.lr_pos = lr_pos; A.lr_tau = lr_tau; A.lr_radius = lr_radius; A.lr_color = lr_color;
+    A.lr_dir = lr_dir; A.lr_aniso = lr_aniso; A.beta1 = beta1; A.beta2 = beta2; A.eps = eps;
+    A.b1c = 1.0f / (1.0f - powf(beta1, (float)t));
+    A.b2c = 1.0f / (1.0f - powf(beta2, (float)t));
+    A.norm = 1.0f / (float)(width * height);
+    A.maxx = (float)(width - 1);
+    A.maxy = (float)(height - 1);
+    A.n = n;
+    if (clear) hipLaunchKernelGGL(k_adam<true>, dim3(div_up(n, 256u)), dim3(256), 0, stream, A);
+    else hipLaunchKernelGGL(k_adam<false>, dim3(div_up(n, 256u)), dim3(256), 0, stream, A);
+    return launch_status("adam_update");
+}
+
+int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    ClearArgs A;
+    for (int j = 0; j < count_arrays; ++j) A.g[j] = arrays[j];
+    A.count_arrays = count_arrays;
+    A.n = n;
+    hipLaunchKernelGGL(k_clear, dim3(div_up(n, 256u)), dim3(256), 0, stream, A);
+    return launch_status("clear");
+}
+
+int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, const float* graw,
+                       const float* gin, float* gout, float* raw_copy, uint32_t n, float lambda, int width,
+                       int height, int ds, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
+    hipLaunchKernelGGL(k_tau_diffuse, dim3(div_up(n, 256u)), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), sites, graw, gin,
+                       gout, raw_copy, n, lambda, width, height, inv_ds);
+    return launch_status("tau_diffuse");
+}
+
+int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const float* const stats[8],
+                 uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+{
+    if (k == 0) return SADH_OK;
+    SplitArgs A;
+    A.sites = sites; A.st = st; A.idx = idx;
+    A.mass = stats[0]; A.ew = stats[2]; A.ewx = stats[3]; A.ewy = stats[4];
+    A.ewxx = stats[5]; A.ewxy = stats[6]; A.ewyy = stats[7];
+    A.target = target; A.k = k; A.cur = cur; A.width = width; A.height = height;
+    hipLaunchKernelGGL(k_split, dim3(div_up(k, 256u)), dim3(256), 0, stream, A);
+    return launch_status("split_sites");
+}
+
+int launch_sse(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
+               hipStream_t stream)
+{
+    uint32_t np = (uint32_t)width * (uint32_t)height;
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
+    hipLaunchKernelGGL(k_sse, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
+    return launch_status("compute_psnr");
+}
+
+int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, count);
+    if (n > 0) hipLaunchKernelGGL(k_count_active, dim3(div_up(n, 256u)), dim3(256), 0, stream, sites, count, n);
+    return launch_status("count_active_sites");
+}
+
+int launch_densify_pairs(const SadSite* sites, const float* mass, const float* energy, SadPair* pairs, uint32_t n,
+                         float min_mass, float alpha, uint32_t count, hipStream_t stream)
+{
+    if (count == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_densify_pairs, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, mass, energy, pairs,
+                       n, min_mass, alpha, count);
+    return launch_status("compute_densify_score_pairs");
+}
+
+int launch_prune_pairs(const SadSite* sites, const float* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
+                       uint32_t count, hipStream_t stream)
+{
+    if (count == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_prune_pairs, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, rdelta, pairs, n,
+                       delta_norm, count);
+    return launch_status("compute_prune_score_pairs");
+}
+
+int launch_write_indices(const SadPair* sorted, uint32_t* idx, uint32_t count, hipStream_t stream)
+{
+    if (count == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_write_indices, dim3(div_up(count, 256u)), dim3(256), 0, stream, sorted, idx, count);
+    return launch_status("write_split_indices");
+}
+
+int launch_prune_sites(SadSite* sites, const uint32_t* idx, uint32_t count, hipStream_t stream)
+{
+    if (count == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_prune_sites, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, idx, count);
+    return launch_status("prune_sites_by_index");
+}
+
+int launch_hilbert_pairs(const SadSite* sites, SadPair* pairs, uint32_t n, uint32_t padded, int width, int height,
+                         int bits, hipStream_t stream)
+{
+    if (padded == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_hilbert_pairs, dim3(div_up(padded, 256u)), dim3(256), 0, stream, sites, pairs, n, padded,
+                       width, height, bits);
+    return launch_status("hilbert_pairs");
+}
+
+int launch_hilbert_order(const SadPair* sorted, uint32_t* order, uint32_t* pos, uint32_t n, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_hilbert_order, dim3(div_up(n, 256u)), dim3(256), 0, stream, sorted, order, pos, n);
+    return launch_status("hilbert_order");
+}
+
+}  // namespace sadh
+
+using namespace sadh;
+
+extern "C" {
+
+int sadh_adam_update(SadSite* sites, SadAdamState* adam, const float* grad_pos_x, const float* grad_pos_y,
+                     const float* grad_log_tau, const float* grad_radius, const float* grad_color_r,
+                     const float* grad_color_g, const float* grad_color_b, const float* grad_dir_x,
+                     const float* grad_dir_y, const float* grad_log_aniso, float lr_pos, float lr_tau,
+                     float lr_radius, float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2,
+                     float eps, uint32_t t, uint32_t site_count, int width, int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && adam && grad_pos_x && grad_pos_y && grad_log_tau && grad_radius && grad_color_r &&
+                     grad_color_g && grad_color_b && grad_dir_x && grad_dir_y && grad_log_aniso,
+                 "adam_update: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0 && t > 0, "adam_update: bad sizes or step");
+    float* const g[10] = {(float*)grad_pos_x, (float*)grad_pos_y, (float*)grad_log_tau, (float*)grad_radius,
+                          (float*)grad_color_r, (float*)grad_color_g, (float*)grad_color_b, (float*)grad_dir_x,
+                          (float*)grad_dir_y, (float*)grad_log_aniso};
+    return launch_adam(sites, adam, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t,
+                       site_count, width, height, false, to_hip(stream));
+}
+
+int sadh_adam_update_clear(SadSite* sites, SadAdamState* adam, float* grad_pos_x, float* grad_pos_y,
+                           float* grad_log_tau, float* grad_radius, float* grad_color_r, float* grad_color_g,
+                           float* grad_color_b, float* grad_dir_x, float* grad_dir_y, float* grad_log_aniso,
+                           float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
+                           float beta1, float beta2, float eps, uint32_t t, uint32_t site_count, int width,
+                           int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && adam && grad_pos_x && grad_pos_y && grad_log_tau && grad_radius && grad_color_r &&
+                     grad_color_g && grad_color_b && grad_dir_x && grad_dir_y && grad_log_aniso,
+                 "adam_update_clear: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0 && t > 0, "adam_update_clear: bad sizes or step");
+    float* const g[10] = {grad_pos_x, grad_pos_y, grad_log_tau, grad_radius, grad_color_r,
+                          grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
+    return launch_adam(sites, adam, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t,
+                       site_count, width, height, true, to_hip(stream));
+}
+
+int sadh_clear_gradients(float* grad_pos_x, float* grad_pos_y, float* grad_log_tau, float* grad_radius,
+                         float* grad_color_r, float* grad_color_g, float* grad_color_b, float* grad_dir_x,
+                         float* grad_dir_y, float* grad_log_aniso, uint32_t count, sadh_stream_t stream)
+{
+    float* const g[10] = {grad_pos_x, grad_pos_y, grad_log_tau, grad_radius, grad_color_r,
+                          grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
+    for (int j = 0; j < 10; ++j) SADH_REQUIRE(g[j], "clear_gradients: null pointer");
+    return launch_clear(g, 10, count, to_hip(stream));
+}
+
+int sadh_clear_buffer(float* buffer, uint32_t count, sadh_stream_t stream)
+{
+    SADH_REQUIRE(buffer, "clear_buffer: null pointer");
+    return launch_clear(&buffer, 1, count, to_hip(stream));
+}
+
+int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite* sites, const float* grad_raw,
+                     const float* grad_in, float* grad_out, uint32_t site_count, float lambda, int width, int height,
+                     int cand_downscale, sadh_stream_t stream)
+{
+    SADH_REQUIRE(cand0 && cand1 && sites && grad_raw && grad_in && grad_out, "tau_diffuse: null pointer");
+    SADH_REQUIRE(grad_in != grad_out, "tau_diffuse: grad_in and grad_out must differ");
+    SADH_REQUIRE(width > 0 && height > 0, "tau_diffuse: bad sizes");
+    return launch_tau_diffuse(cand0, cand1, sites, grad_raw, grad_in, grad_out, nullptr, site_count, lambda, width,
+                              height, cand_downscale, to_hip(stream));
+}
+
+int sadh_compute_densify_score_pairs(const SadSite* sites, const float* mass, const float* energy, SadPair* pairs,
+                                     uint32_t site_count, float min_mass, float score_alpha, uint32_t pair_count,
+                                     sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && mass && energy && pairs, "compute_densify_score_pairs: null pointer");
+    SADH_REQUIRE(site_count <= pair_count, "compute_densify_score_pairs: site_count > pair_count");
+    return launch_densify_pairs(sites, mass, energy, pairs, site_count, min_mass, score_alpha, pair_count,
+                                to_hip(stream));
+}
+
+int sadh_compute_prune_score_pairs(const SadSite* sites, const float* removal_delta, SadPair* pairs,
+                                   uint32_t site_count, float delta_norm, uint32_t pair_count, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && removal_delta && pairs, "compute_prune_score_pairs: null pointer");
+    SADH_REQUIRE(site_count <= pair_count, "compute_prune_score_pairs: site_count > pair_count");
+    return launch_prune_pairs(sites, removal_delta, pairs, site_count, delta_norm, pair_count, to_hip(stream));
+}
+
+int sadh_write_split_indices(const SadPair* sorted_pairs, uint32_t* indices, uint32_t count, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sorted_pairs && indices, "write_split_indices: null pointer");
+    return launch_write_indices(sorted_pairs, indices, count, to_hip(stream));
+}
+
+int sadh_split_sites(SadSite* sites, SadAdamState* adam, const uint32_t* split_indices, uint32_t num_to_split,
+                     const float* mass, const float* err_w, const float* err_wx, const float* err_wy,
+                     const float* err_wxx, const float* err_wxy, const float* err_wyy, uint32_t current_site_count,
+                     const float* target, int width, int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && adam && split_indices && mass && err_w && err_wx && err_wy && err_wxx && err_wxy &&
+                     err_wyy && target,
+                 "split_sites: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0, "split_sites: bad sizes");
+    const float* const st[8] = {mass, nullptr, err_w, err_wx, err_wy, err_wxx, err_wxy, err_wyy};
+    return launch_split(sites, adam, split_indices, num_to_split, st, current_site_count, target, width, height,
+                        to_hip(stream));
+}
+
+int sadh_prune_sites_by_index(SadSite* sites, const uint32_t* indices, uint32_t count, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && indices, "prune_sites_by_index: null pointer");
+    return launch_prune_sites(sites, indices, count, to_hip(stream));
+}
+
+int sadh_hilbert_pairs(const SadSite* sites, SadPair* pairs, uint32_t site_count, uint32_t padded_count, int width,
+                       int height, int bits, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && pairs && width > 0 && height > 0, "hilbert_pairs: bad arguments");
+    SADH_REQUIRE(site_count <= padded_count && bits >= 1 && bits <= 16, "hilbert_pairs: bad sizes");
+    return launch_hilbert_pairs(sites, pairs, site_count, padded_count, width, height, bits, to_hip(stream));
+}
+
+int sadh_hilbert_order(const SadPair* sorted_pairs, uint32_t* order, uint32_t* pos, uint32_t site_count,
+                       sadh_stream_t stream)
+{
+    SADH_REQUIRE(sorted_pairs && order && pos, "hilbert_order: null pointer");
+    return launch_hilbert_order(sorted_pairs, order, pos, site_count, to_hip(stream));
+}
+
+int sadh_compute_psnr(const float* rendered, const float* target, const float* mask, float* mse_accum, int width,
+                      int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(rendered && target && mask && mse_accum && width > 0 && height > 0, "compute_psnr: bad arguments");
+    return launch_sse(rendered, target, mask, mse_accum, width, height, to_hip(stream));
+}
+
+int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && count, "count_active_sites: null pointer");
+    return launch_count_active(sites, count, site_count, to_hip(stream));
+}
+
+}  // extern "C"

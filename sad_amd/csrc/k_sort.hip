@@ -1,0 +1,146 @@
+/*
+ * k_sort.hip - stable LSD radix sort of (key, value) uint32 pairs by key,
+ * 8-bit digits (launchRadixSortUInt2, radix_sort.cu:279-336).
+ *
+ * Per digit pass, three launches:
+ *   1. per-tile digit histogram (tile = 1024 pairs = 4 chunks of 256),
+ *      written digit-major: hist[digit * tiles + tile];
+ *   2. one-workgroup exclusive scan of hist (1024 threads), which turns every
+ *      entry into the global output offset of (digit, tile);
+ *   3. scatter: each 256-pair chunk is ranked stably with wave64 ballots (the
+ *      8 digit bits are matched with 8 ballots, rank = popcount of the matching
+ *      lower lanes), waves are ordered through LDS counts, and a per-digit
+ *      running offset carries the order from chunk to chunk.
+ * The reference ranks each element with an O(256) loop over the chunk
+ * (radix_sort.cu:244-253); the ballot match is O(8) per element.
+ * Output order is exactly that of a stable sort by key, so it is bit-exact
+ * against any stable CPU sort.
+ */
+#include "sad_device.h"
+
+namespace sadh {
+
+constexpr uint32_t SORT_TILE = 1024u;
+
+__global__ __launch_bounds__(256) void k_sort_hist(const uint2* __restrict__ in, uint32_t* __restrict__ hist,
+                                                   uint32_t n, uint32_t tiles, uint32_t shift)
+{
+    __shared__ uint32_t h[256];
+    uint32_t t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    uint32_t base = blockIdx.x * SORT_TILE;
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; ++c) {
+        uint32_t i = base + c * 256u + t;
+        if (i < n) atomicAdd(&h[(in[i].x >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[t * tiles + blockIdx.x] = h[t];
+}
+
+/* Exclusive scan of len words in place by one 1024-thread workgroup. */
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ data, uint32_t len)
+{
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
+    uint32_t t = threadIdx.x;
+    uint32_t per = (len + 1023u) / 1024u;
+    uint32_t b = t * per, e = min(b + per, len);
+    uint32_t s = 0u;
+    for (uint32_t i = b; i < e; ++i) s += data[i];
+    /* inclusive scan of s across the workgroup: wave scan, then wave totals */
+    uint32_t lane = t & 63u, w = t >> 6;
+    uint32_t v = s;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        uint32_t o = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= off) v += o;
+    }
+    if (lane == 63u) wsum[w] = v;
+    __syncthreads();
+    if (t < 16u) {
+        uint32_t acc = 0u;
+        for (uint32_t j = 0; j < t; ++j) acc += wsum[j];
+        part[t] = acc;
+    }
+    __syncthreads();
+    uint32_t run = part[w] + v - s;   /* exclusive prefix of this thread's segment */
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t x = data[i];
+        data[i] = run;
+        run += x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const uint2* __restrict__ in, uint2* __restrict__ out,
+                                                      const uint32_t* __restrict__ offs, uint32_t n, uint32_t tiles,
+                                                      uint32_t shift)
+{
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wc[4][256];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    running[t] = offs[t * tiles + blockIdx.x];
+    const unsigned long long lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (uint32_t c = 0; c < 4u; ++c) {
+        wc[0][t] = 0u; wc[1][t] = 0u; wc[2][t] = 0u; wc[3][t] = 0u;
+        uint32_t i = base + c * 256u + t;
+        bool valid = i < n;
+        uint2 v = valid ? in[i] : make_uint2(0u, 0u);
+        uint32_t d = (v.x >> shift) & 0xFFu;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t bit = 0; bit < 8u; ++bit) {
+            bool on = (d >> bit) & 1u;
+            unsigned long long bal = __ballot(on);
+            peers &= on ? bal : ~bal;
+        }
+        __syncthreads();
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        bool leader = valid && (peers >> lane) == 1ull;   /* highest lane of its digit group */
+        if (leader) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t o = running[d] + rank;
+            for (uint32_t j = 0; j < w; ++j) o += wc[j][d];
+            out[o] = v;
+        }
+        __syncthreads();
+        running[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        __syncthreads();
+    }
+}
+
+int launch_radix_sort(SadPair* data, SadPair* scratch, uint32_t* hist, uint32_t n, uint32_t max_key_exclusive,
+                      hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    uint32_t tiles = div_up(n, SORT_TILE);
+    int passes = (max_key_exclusive <= (1u << 16)) ? 2 : 4;
+    uint2* a = reinterpret_cast<uint2*>(data);
+    uint2* b = reinterpret_cast<uint2*>(scratch);
+    for (int p = 0; p < passes; ++p) {
+        uint32_t shift = 8u * (uint32_t)p;
+        hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(256), 0, stream, a, hist, n, tiles, shift);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, stream, hist, 256u * tiles);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(256), 0, stream, a, b, hist, n, tiles, shift);
+        int rc = launch_status("radix_sort_pairs");
+        if (rc) return rc;
+        uint2* tmp = a; a = b; b = tmp;
+    }
+    return SADH_OK;
+}
+
+}  // namespace sadh
+
+using namespace sadh;
+
+extern "C" int sadh_radix_sort_pairs(SadPair* data, SadPair* scratch, uint32_t* hist, uint32_t* block_sums,
+                                     uint32_t padded_count, uint32_t max_key_exclusive, sadh_stream_t stream)
+{
+    (void)block_sums;
+    SADH_REQUIRE(data && scratch && hist, "radix_sort_pairs: null pointer");
+    SADH_REQUIRE(data != scratch, "radix_sort_pairs: scratch must not alias data");
+    return launch_radix_sort(data, scratch, hist, padded_count, max_key_exclusive, to_hip(stream));
+}

@@ -322,6 +322,67 @@ SAD_HD float sad_packed_key(SadPackedSite p, float ux, float uy, float inv_scale
     return tau * d;
 }
 
+/* ---------------------------------------------------- gradient terms --
+ * Derivative of the loss through one (pixel, candidate) pair
+ * (backends/cuda/src/gradients.cu:352-420, sad_bindings.cu:145-193), given the
+ * site terms tau = e^log_tau, rn = r/S, dir, l1 = e^a, l2 = 1/max(l1, 1e-8),
+ * the pixel (ux, uy), the candidate weight w, dL/dpred g and the blended colour
+ * pred.  c[] receives the ten contributions in Site field order
+ * {pos.x, pos.y, log_tau, radius, r, g, b, dir.x, dir.y, log_aniso}:
+ *   dlogit = w g.(col - pred)
+ *   dpos   = dlogit tau S^-2 d2^-1/2 (l2 diff + (l1 - l2) dir proj)
+ *   dlog_tau = -dlogit dmix tau,  dradius = dlogit tau / S
+ *   dlog_aniso = -dlogit tau d2^-1/2 (l1 proj^2 - l2 perp^2) S^-2 / 2
+ *   ddir   = -dlogit tau d2^-1/2 (l1 - l2) proj S^-2 diff
+ *   dcol   = w g.
+ * The CPU oracle and the GPU kernels evaluate this one function, so per-term
+ * values agree bit for bit and parity is limited to summation order; the
+ * function itself is checked against finite differences of the render. */
+SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
+                           float cr, float cg, float cb, float ux, float uy, float w, float gr, float gg, float gb,
+                           float pr, float pg, float pb, float iss, float inv_scale, float c[10])
+{
+    c[4] = w * gr;
+    c[5] = w * gg;
+    c[6] = w * gb;
+    float dlogit = c[4] * (cr - pr) + c[5] * (cg - pg) + c[6] * (cb - pb);
+    float dx = ux - px, dy = uy - py;
+    float diff2 = fmaf(dx, dx, dy * dy);
+    float proj = fmaf(dirx, dx, diry * dy);
+    float proj2 = proj * proj;
+    float perp2 = fmaxf(diff2 - proj2, 0.0f);
+    float d2s = fmaxf(fmaf(l1, proj2, l2 * perp2) * iss, 1e-8f);
+    float sq = sqrtf(d2s);
+    float isd = 1.0f / sq;
+    float dmix = sq - rn;
+    float dl12 = l1 - l2;
+    float cpos = dlogit * (tau * iss * isd);
+    c[0] = cpos * fmaf(dl12 * dirx, proj, l2 * dx);
+    c[1] = cpos * fmaf(dl12 * diry, proj, l2 * dy);
+    c[2] = -dlogit * dmix * tau;
+    c[3] = dlogit * tau * inv_scale;
+    float hq = -dlogit * tau * 0.5f * isd;
+    c[9] = hq * (fmaf(l1, proj2, -(l2 * perp2)) * iss);
+    float cd = 2.0f * dl12 * proj * iss;
+    c[7] = hq * (cd * dx);
+    c[8] = hq * (cd * dy);
+}
+
+/* removalDeltaForSite (gradients.cu:82-100): change of |pred - tgt|^2 when the
+ * site's summed weight wt is removed and the rest renormalised; 1e6 when
+ * 1 - wt < 1e-4. */
+SAD_HD float sad_removal_delta(float pr, float pg, float pb, float tr, float tg, float tb, float cr, float cg,
+                               float cb, float wt)
+{
+    float keep = 1.0f - wt;
+    if (keep < 1e-4f) return 1.0e6f;
+    float k0 = (pr - wt * cr) / keep - tr;
+    float k1 = (pg - wt * cg) / keep - tg;
+    float k2 = (pb - wt * cb) / keep - tb;
+    float d0 = pr - tr, d1 = pg - tg, d2 = pb - tb;
+    return (k0 * k0 + k1 * k1 + k2 * k2) - (d0 * d0 + d1 * d1 + d2 * d2);
+}
+
 SAD_HD SadPackedSite sad_pack_site(const SadSite* s)
 {
     SadPackedSite p;

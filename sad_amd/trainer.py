@@ -1,0 +1,72 @@
+"""Host-side driver of the native fit and render loops (sadh_fit,
+sadh_render_sites): numpy in, numpy out."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .config import fit_options, to_struct
+
+
+def fit_plan(opts: dict, width: int, height: int) -> dict:
+    s = to_struct(opts)
+    d, p, cap, pairs = C.c_float(), C.c_float(), C.c_int(), C.c_int()
+    _lib.call("sadh_fit_plan", C.byref(s), width, height, C.byref(d), C.byref(p), C.byref(cap), C.byref(pairs))
+    return dict(densify_percentile=d.value, prune_percentile=p.value, capacity=cap.value, pair_count=pairs.value)
+
+
+def fit(target: np.ndarray, sites: np.ndarray, opts: dict | None = None, mask: np.ndarray | None = None,
+        **over) -> dict:
+    """Run the fit on the current HIP device.
+
+    target float32 [H][W][3]; sites float32 [N][10] initial sites.  Returns a
+    dict with the final sites [capacity][10], logged PSNRs and the SadhFitResult
+    fields."""
+    o = dict(opts or fit_options())
+    o.update(over)
+    h, w, _ = target.shape
+    if o["init_mode"] == 0:
+        o["n_sites"] = int(sites.shape[0])
+    plan = fit_plan(o, w, h)
+    cap = max(plan["capacity"], sites.shape[0])
+    buf = np.zeros((cap, 10), np.float32)
+    buf[:, 0:2] = -1.0
+    buf[: sites.shape[0]] = sites
+    tgt = np.ascontiguousarray(target, np.float32)
+    msk = None if mask is None else np.ascontiguousarray(mask, np.float32)
+    logs = np.zeros(64, np.float32)
+    res = _lib.FitResult()
+    trace = np.zeros(max(1, o["iterations"]), np.float32) if o.get("profile_gradients") else None
+    if trace is not None:
+        res.grad_ms_trace = trace.ctypes.data_as(C.POINTER(C.c_float))
+        res.grad_ms_trace_cap = trace.size
+    s = to_struct(o)
+    _lib.call("sadh_fit", C.byref(s), _lib.ptr(tgt), _lib.ptr(msk), w, h, _lib.ptr(buf), cap, _lib.ptr(logs),
+              len(logs), C.byref(res))
+    out = {k: getattr(res, k) for k, _ in _lib.FitResult._fields_ if not k.startswith("grad_ms_trace")}
+    out["grad_ms_trace"] = trace
+    out["sites"] = buf
+    out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
+    out["plan"] = plan
+    return out
+
+
+def render_sites(sites: np.ndarray, width: int, height: int, cand_passes: int = 16, use_jfa: bool = True,
+                 jfa_rounds: int = 1, cand_downscale: int = 1, radius_scale: float = 64.0, radius_probes: int = 0,
+                 inject_count: int = 4, hilbert_window: int = 32, hilbert_probes: int = 1) -> dict:
+    """Render-only job: JFA + VPT candidate build, then the softmax render
+    (renderVoronoiFromSites, backends/cuda/src/sad_cuda.cu:1678-1858)."""
+    s = np.ascontiguousarray(sites, np.float32)
+    out = np.zeros((height, width, 3), np.float32)
+    cms, rms = C.c_float(), C.c_float()
+    _lib.call("sadh_render_sites", _lib.ptr(s), s.shape[0], width, height, cand_passes, int(use_jfa), jfa_rounds,
+              cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
+              _lib.ptr(out), C.byref(cms), C.byref(rms))
+    return dict(image=out, cand_ms=cms.value, render_ms=rms.value)
+
+
+def psnr(a: np.ndarray, b: np.ndarray) -> float:
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return 100.0 if mse <= 0 else 20.0 * np.log10(1.0 / np.sqrt(mse))

@@ -1,0 +1,54 @@
+"""The C ABI library: loads on a CPU-only host, exports every entry point that
+include/sad_hip.h declares, and rejects bad arguments with a status code and a
+message before touching a device."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sad_amd import _lib, config
+
+
+def test_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 28
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing
+    assert set(_lib.SIGNATURES) <= set(declared)
+    assert lib.sadh_version().decode().startswith("sad_hip")
+
+
+@pytest.mark.parametrize("name,args", [
+    ("sadh_render_voronoi", [None, None, None, None, 1.0, 1, 4, 4, 4, 4, None]),
+    ("sadh_update_candidates", [None] * 5 + [1, 0, 1.0, 0, 64.0, 0, 4, None, None, 0, 0, 1, 4, 4, 4, 4, None]),
+    ("sadh_radix_sort_pairs", [None, None, None, None, 16, 0xFFFFFFFF, None]),
+    ("sadh_clear_buffer", [None, 4, None]),
+])
+def test_null_pointers_are_rejected(name, args):
+    lib = _lib.load()
+    rc = getattr(lib, name)(*args)
+    assert rc == -1
+    assert "null" in lib.sadh_last_error().decode() or "bad" in lib.sadh_last_error().decode()
+
+
+def test_ssim_loss_is_refused():
+    lib = _lib.load()
+    z = C.c_void_p(16)
+    rc = lib.sadh_compute_gradients_tiled(*([z] * 15), z, 1.0, 1, z, 0, 0.5, 2, 2, 2, 2, None)
+    assert rc == -2 and "SSIM" in lib.sadh_last_error().decode()
+
+
+def test_fit_plan_and_capacity_error():
+    from sad_amd import trainer
+    o = config.fit_options(n_sites=16384, target_bpp=1.0, iterations=2000, densify_freq=100, prune_freq=100,
+                           max_sites=262144)
+    p = trainer.fit_plan(o, 1024, 1024)
+    assert p["capacity"] == 262144 and p["pair_count"] == 262144
+    assert 0 < p["densify_percentile"] < p["prune_percentile"] < 0.95
+    s = config.to_struct(o)
+    sites = np.zeros((10, 10), np.float32)
+    res = _lib.FitResult()
+    rc = _lib.load().sadh_fit(C.byref(s), sites.ctypes.data, None, 1024, 1024, sites.ctypes.data, 10, None, 0,
+                              C.byref(res))
+    assert rc == -3 and "plan needs" in _lib.load().sadh_last_error().decode()
