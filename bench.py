@@ -10,7 +10,9 @@ value = sum(W*H/1e6 * iterations) / sum(fit-loop seconds) over the timed steps;
 the loop time starts after the target and initial sites are resident in HBM.
 roofline: the tiled gradient kernel (the dominant kernel), algorithmic bytes
 48*P + 120*N per launch (+8*N on prune iterations) over its average duration,
-measured with HIP events on its stream inside the fit.
+measured with HIP events on its stream inside the fit; traffic: HBM bytes per
+launch of the same kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE passes, scaled
+by a copy kernel of known size (profiles/k_pixel_grad_<workload>_traffic.md).
 cpu_baseline: the CPU oracle (C, OpenMP) on the first iterations of the same
 fit, as a bounded sample.
 """
@@ -24,6 +26,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def pmc_traffic(workload: str):
+    """HBM bytes per gradient launch from the committed rocprofv3 PMC summary.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE of the gradient kernel over the
+    same fit, each scaled by what the counter reads for a 256 MiB copy.
+    Returns None when no summary exists for the workload.
+    """
+    path = os.path.join(ROOT, "profiles", f"k_pixel_grad_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def main() -> int:
@@ -104,7 +124,8 @@ def main() -> int:
                    "iterations": iters, "final_sites": last["actual_sites"], "active_sites": last["active_count"],
                    "final_psnr_db": round(last["final_psnr"], 3), "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload),
+                     "algorithmic_bytes_per_launch": int(gbytes / max(1, glaunch)),
                      "kernel": "k_pixel<MODE_GRAD> (compute_gradients_tiled)",
                      "avg_launch_us": round(1e3 * gms / max(1, glaunch), 3)},
         "cpu_baseline": cpu,
