@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from tests.scenes import converged_field, hilbert_tables, iss_of, random_sites
+from tests.tolerance import Rel
 
 pytestmark = pytest.mark.gpu
 
@@ -187,22 +188,8 @@ def test_render_edge_cases(gpu, orc):
     assert np.abs(gpu.get(out) - ref).max() <= 1e-5
 
 
-class _Rel(float):
-    """max of |got - ref| / (1e-4 |ref| + 1e-5 sum|terms|) (<= 1 passes), with
-    the worst element's values for the failure message."""
-    def __new__(cls, got, ref, ab):
-        err = np.abs(got.astype(np.float64) - ref) / (1e-4 * np.abs(ref) + 1e-5 * ab + 1e-30)
-        i = np.unravel_index(int(np.argmax(err)), err.shape)
-        obj = super().__new__(cls, float(err[i]))
-        obj.info = f"worst at {tuple(int(v) for v in i)}: gpu {float(got[i])!r} ref {float(ref[i])!r} sum|terms| {float(ab[i])!r}"
-        return obj
-
-    def __repr__(self):
-        return f"{float(self):.4g} ({self.info})"
-
-
 def _rel_ok(got, ref, ab):
-    return _Rel(got, ref, ab)
+    return Rel(got, ref, ab)
 
 
 @pytest.mark.parametrize("removal", [False, True])
