@@ -18,11 +18,15 @@
  *      per (slot, pixel) with an integer LDS atomic; overflow candidates are
  *      differentiated right here and go to global memory with float atomics.
  *   4. a scan of the per-slot counts turns ranks into positions of a per-slot
- *      entry list (pixel, candidate rank).
- *   5. site-centric accumulation: 8 lanes per slot walk the slot's entries,
- *      differentiate each (pixel, candidate) pair and sum in registers; an
- *      8-lane DPP reduction then gives one global float atomic per
- *      (site, parameter) and tile.
+ *      entry list (pixel, candidate rank), and cuts every slot's list into
+ *      chunks of CHUNK entries.
+ *   5. chunk-parallel accumulation: 8 lanes per chunk differentiate its
+ *      (pixel, candidate) pairs and sum in registers; an 8-lane DPP reduction
+ *      leaves one partial per chunk in LDS; the partials of each slot are then
+ *      summed (one thread per site and parameter) into one global float atomic
+ *      per (site, parameter) and tile.  Chunking balances the work: the entry
+ *      counts of a tile's sites range from a few to all 256 pixels, and giving
+ *      whole slots to lane groups left most lanes idle behind the largest one.
  * Why not LDS float atomics (the reference's tile hash, gradients.cu:422-459):
  * on gfx950, ds_add_f32 measured ~190 cycles per wave instruction per CU
  * against ~5 for ds_add_u32 and ~8 for a read-add-write, and the hashed
@@ -37,6 +41,13 @@
  * sad_grad_terms, the function the oracle evaluates, so GPU and CPU differ only
  * in the order of the fp32 sums.
  */
+#include <type_traits>
+
+/* derivative terms: hardware rsq (~1 ulp) for sqrt(d2) and 1/sqrt(d2); the
+ * softmax forward keeps the exact sqrt and exp */
+#ifndef SAD_EXACT_GRAD_MATH
+#define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (isd) = __builtin_amdgcn_rsqf(x); (sq) = (x) * (isd); } while (0)
+#endif
 #include "sad_device.h"
 
 namespace sadh {
@@ -47,22 +58,45 @@ constexpr uint16_t SLOT_INVALID = 0xFFFFu;   /* id >= site count or pixel not pr
 constexpr uint16_t SLOT_OVERFLOW = 0xFFFEu;  /* valid id without a hash slot */
 constexpr uint16_t ENTRY_FIRST = 0x8000u;    /* entry flag: first rank of its site in the pixel */
 
+#ifndef SAD_CHUNK
+#define SAD_CHUNK 24
+#endif
+constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (3 per lane of an 8-lane group) */
+constexpr uint8_t SLOT8_NONE = 0xFFu;        /* per-pixel table: candidate not accumulated via a slot */
+
+/* values accumulated per site: 10 gradients (+ removal delta), or 8 statistics */
+template <int MODE, bool REMOVAL>
+struct NumValues {
+    static constexpr int value = (MODE == 3) ? 8 : (REMOVAL ? 11 : 10);
+};
+
+/* LDS of every mode: the tile's site hash and staged site terms */
 template <int H>
-struct TileLds {
+struct TileSites {
     uint32_t key[H];
     float4 p0[H];              /* pos.x, pos.y, tau, r/S      (pos.x < 0: inactive) */
     float4 p1[H];              /* dir.x, dir.y, e^a, 1/e^a */
     float4 p2[H];              /* r, g, b, 1/max(e^a, 1e-8) */
     uint32_t count[H];         /* entries per slot */
-    uint32_t start[H];         /* first entry of each slot in `list` */
     uint16_t occ[H];           /* dense list of occupied slots */
     uint32_t n_occ;
-    uint16_t slot[256][8];     /* per pixel: slot of each candidate rank (INVALID if unused) */
+};
+
+/* LDS of the backward modes: per-pixel data, entry lists, chunk partials */
+template <int H, bool REMOVAL, int NV>
+struct TileLds : TileSites<H> {
+    static constexpr uint32_t MAX_CHUNKS = 2048u / CHUNK + (uint32_t)H;
+    uint32_t start[H];         /* first entry of each slot in `list` */
+    uint16_t cstart[H];        /* first chunk of each slot */
+    uint32_t n_chunks;
+    uint8_t slot[256][8];      /* per pixel: slot of each candidate rank (SLOT8_NONE if unused) */
     float w[256][8];           /* per pixel: softmax weights */
     float4 px[256];            /* per pixel: pred.rgb, err (STATS) */
     float4 pg[256];            /* per pixel: dL/dpred.rgb */
-    float4 pt[256];            /* per pixel: target.rgb (removal deltas) */
+    float4 pt[REMOVAL ? 256 : 1];  /* per pixel: target.rgb (removal deltas) */
     uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST) */
+    uint8_t chunk_slot[MAX_CHUNKS];
+    float part[MAX_CHUNKS][NV];    /* per-chunk partial sums */
 };
 
 struct SiteTerms {
@@ -82,8 +116,8 @@ __device__ __forceinline__ SiteTerms site_terms(const SadSite* __restrict__ site
     return t;
 }
 
-template <int H>
-__device__ __forceinline__ uint16_t hash_insert(TileLds<H>& L, uint32_t id)
+template <class LDS, int H>
+__device__ __forceinline__ uint16_t hash_insert(LDS& L, uint32_t id)
 {
     uint32_t h = (id * 2654435761u) & (uint32_t)(H - 1);
 #pragma unroll 1
@@ -141,7 +175,10 @@ struct PixelArgs {
 template <int MODE, bool REMOVAL, int H>
 __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
-    __shared__ TileLds<H> L;
+    constexpr int NV = NumValues<MODE, REMOVAL>::value;
+    static_assert(H <= 128, "per-pixel slot table holds 7-bit slots");
+    using Lds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>, TileLds<H, REMOVAL, NV>>::type;
+    __shared__ Lds L;
     const uint32_t tid = threadIdx.x;
     const uint32_t tile = xcd_tile(blockIdx.x, A.ntiles);
     const int tx0 = (int)(tile % A.tiles_x) * 16, ty0 = (int)(tile / A.tiles_x) * 16;
@@ -172,7 +209,7 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
         ids[0] = a.x; ids[1] = a.y; ids[2] = a.z; ids[3] = a.w;
         ids[4] = b.x; ids[5] = b.y; ids[6] = b.z; ids[7] = b.w;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) slots[k] = (act && ids[k] < A.n) ? hash_insert<H>(L, ids[k]) : SLOT_INVALID;
+        for (int k = 0; k < 8; ++k) slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
     }
     __syncthreads();
 
@@ -246,15 +283,14 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
         }
     }
 
-    if (MODE == MODE_RENDER) {
+    if constexpr (MODE == MODE_RENDER) {
         if (inb) {
             float* o = A.out + 3u * pix;
             o[0] = has ? pr : 0.0f;
             o[1] = has ? pgr : 0.0f;
             o[2] = has ? pb : 0.0f;
         }
-        return;
-    }
+    } else {
 
     /* dL/dpred and the per-pixel data the site-centric pass reads */
     const bool contrib = act && has;
@@ -276,14 +312,15 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
     L.pg[tid] = make_float4(gr, gg, gb, 0.0f);
     if (REMOVAL) L.pt[tid] = make_float4(tr, tg, tb, 0.0f);
     uint32_t rank[8];
+    uint8_t sl[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         L.w[tid][k] = w[k];
         const uint16_t s = slots[k];
         const bool use = contrib && valid[k];
-        rank[k] = 0u;
-        if (use && s < (uint16_t)H) rank[k] = atomicAdd(&L.count[s], 1u);
-        L.slot[tid][k] = (use && s < (uint16_t)H) ? s : SLOT_INVALID;
+        sl[k] = (use && s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
+        rank[k] = (sl[k] != SLOT8_NONE) ? atomicAdd(&L.count[s], 1u) : 0u;
+        L.slot[tid][k] = sl[k];
         if (use && s == SLOT_OVERFLOW) {
             /* an id without a hash slot: finish its contribution here, in global memory */
             SiteTerms t = site_terms(A.sites, ids[k], A.inv_scale);
@@ -322,51 +359,60 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
     }
     __syncthreads();
 
-    /* stage 4: list offsets = exclusive scan of the per-slot counts (wave 0) */
+    /* stage 4 (wave 0): entry offsets and chunk offsets = exclusive scans of the
+     * per-slot entry and chunk counts; chunk -> slot table */
     if (tid < 64u) {
-        uint32_t run = 0u;
+        uint32_t run = 0u, crun = 0u;
         for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
-            uint32_t c = L.count[s0 + tid];
-            uint32_t v = c;
+            const uint32_t s = s0 + tid;
+            const uint32_t c = L.count[s];
+            const uint32_t nc = (c + CHUNK - 1u) / CHUNK;
+            uint32_t v = c, vc = nc;
 #pragma unroll
             for (uint32_t off = 1; off < 64u; off <<= 1) {
                 uint32_t o = (uint32_t)__shfl_up((int)v, off);
-                if (tid >= off) v += o;
+                uint32_t oc = (uint32_t)__shfl_up((int)vc, off);
+                if (tid >= off) { v += o; vc += oc; }
             }
-            L.start[s0 + tid] = run + v - c;
+            const uint32_t cs = crun + vc - nc;
+            L.start[s] = run + v - c;
+            L.cstart[s] = (uint16_t)cs;
+            for (uint32_t j = 0; j < nc; ++j) L.chunk_slot[cs + j] = (uint8_t)s;
             run += (uint32_t)__shfl((int)v, 63);
+            crun += (uint32_t)__shfl((int)vc, 63);
         }
+        if (tid == 0u) L.n_chunks = crun;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const uint16_t s = L.slot[tid][k];
-        if (s != SLOT_INVALID) {
+        const uint8_t s = sl[k];
+        if (s != SLOT8_NONE) {
             bool first = true;
 #pragma unroll
-            for (int j = 0; j < k; ++j) first = first && (L.slot[tid][j] != s);
+            for (int j = 0; j < k; ++j) first = first && (sl[j] != s);
             L.list[L.start[s] + rank[k]] = (uint16_t)((tid << 3) | (uint32_t)k | (first ? ENTRY_FIRST : 0u));
         }
     }
     __syncthreads();
 
-    /* stage 5: 8 lanes per occupied slot walk its entry list */
-    const uint32_t n_occ = L.n_occ;
+    /* stage 5: 8 lanes per chunk of CHUNK entries; one partial per chunk */
+    const uint32_t n_chunks = L.n_chunks;
     const uint32_t part = tid & 7u;
-    for (uint32_t base = 0; base < n_occ; base += 32u) {
-        const uint32_t u = base + (tid >> 3);
-        const bool live = u < n_occ;
-        const uint32_t s = live ? (uint32_t)L.occ[u] : 0u;
-        const uint32_t beg = live ? L.start[s] : 0u;
-        const uint32_t end = live ? beg + L.count[s] : 0u;
+    for (uint32_t base = 0; base < n_chunks; base += 32u) {
+        const uint32_t q = base + (tid >> 3);
+        const bool live = q < n_chunks;
+        const uint32_t s = live ? (uint32_t)L.chunk_slot[q] : 0u;
+        const uint32_t sbeg = L.start[s];
+        const uint32_t beg = sbeg + (q - (uint32_t)L.cstart[s]) * CHUNK;
+        const uint32_t end = live ? min(beg + CHUNK, sbeg + L.count[s]) : 0u;
         SiteTerms t;
         t.p0 = L.p0[s];
         t.p1 = L.p1[s];
         t.p2 = L.p2[s];
-        float acc[10];
+        float acc[NV];
 #pragma unroll
-        for (int j = 0; j < 10; ++j) acc[j] = 0.0f;
-        float dsum = 0.0f;
+        for (int j = 0; j < NV; ++j) acc[j] = 0.0f;
         for (uint32_t e = beg + part; e < end; e += 8u) {
             const uint32_t ent = L.list[e];
             const uint32_t p = (ent & 0x7FFFu) >> 3, k = ent & 7u;
@@ -377,7 +423,7 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
                 if (ent & ENTRY_FIRST) {
                     float wt = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) wt += (L.slot[p][j] == (uint16_t)s) ? L.w[p][j] : 0.0f;
+                    for (int j = 0; j < 8; ++j) wt += (L.slot[p][j] == (uint8_t)s) ? L.w[p][j] : 0.0f;
                     if (wt > 0.0f) {
                         if (MODE == MODE_STATS) {
                             const float er = pp.w;
@@ -387,7 +433,8 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
                             acc[5] += we * ex * ex; acc[6] += we * ex * ey; acc[7] += we * ey * ey;
                         } else {
                             const float4 tq = L.pt[p];
-                            dsum += sad_removal_delta(pp.x, pp.y, pp.z, tq.x, tq.y, tq.z, t.p2.x, t.p2.y, t.p2.z, wt);
+                            acc[NV - 1] += sad_removal_delta(pp.x, pp.y, pp.z, tq.x, tq.y, tq.z, t.p2.x, t.p2.y,
+                                                             t.p2.z, wt);
                         }
                     }
                 }
@@ -399,22 +446,37 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
                 for (int j = 0; j < 10; ++j) acc[j] += c[j];
             }
         }
-        constexpr int NOUT = (MODE == MODE_STATS) ? 8 : 10;
+        /* every lane of the group ends with the same total; lane `part` stores values part and part + 8 */
 #pragma unroll
-        for (int j = 0; j < NOUT; ++j) acc[j] = group8_sum(acc[j]);
-        if (MODE == MODE_GRAD && REMOVAL) dsum = group8_sum(dsum);
-        if (live && part == 0u && t.p0.x >= 0.0f) {
-            const uint32_t id = L.key[s];
-            if (MODE == MODE_BWD) {
+        for (int j = 0; j < NV; ++j) acc[j] = group8_sum(acc[j]);
+        if (live) {
 #pragma unroll
-                for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, acc[j]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < NOUT; ++j) unsafeAtomicAdd(A.g[j] + id, acc[j]);
-                if (MODE == MODE_GRAD && REMOVAL && dsum != 0.0f) unsafeAtomicAdd(A.removal + id, dsum);
-            }
+            for (int j = 0; j < NV; ++j)
+                if ((uint32_t)(j & 7) == part) L.part[q][j] = acc[j];
         }
     }
+    __syncthreads();
+
+    /* per (site, value): sum the slot's chunk partials, one global atomic */
+    const uint32_t n_occ = L.n_occ;
+    for (uint32_t i = tid; i < n_occ * (uint32_t)NV; i += 256u) {
+        const uint32_t u = i / (uint32_t)NV, j = i - u * (uint32_t)NV;
+        const uint32_t s = L.occ[u];
+        const uint32_t c = L.count[s];
+        if (c == 0u || !(L.p0[s].x >= 0.0f)) continue;
+        const uint32_t q0 = L.cstart[s], q1 = q0 + (c + CHUNK - 1u) / CHUNK;
+        float v = 0.0f;
+        for (uint32_t q = q0; q < q1; ++q) v += L.part[q][j];
+        const uint32_t id = L.key[s];
+        if (MODE == MODE_BWD) {
+            unsafeAtomicAdd(A.out + 10u * id + j, v);
+        } else if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
+            if (v != 0.0f) unsafeAtomicAdd(A.removal + id, v);
+        } else {
+            unsafeAtomicAdd(A.g[j] + id, v);
+        }
+    }
+    }  /* backward modes */
 }
 
 /* ------------------------------------------------------------ launchers -- */

@@ -338,6 +338,14 @@ SAD_HD float sad_packed_key(SadPackedSite p, float ux, float uy, float inv_scale
  * The CPU oracle and the GPU kernels evaluate this one function, so per-term
  * values agree bit for bit and parity is limited to summation order; the
  * function itself is checked against finite differences of the render. */
+/* sqrt(d2) and 1/sqrt(d2) of the derivative terms.  The terms are summed over
+ * pixels and compared with a relative tolerance, so a device build may use the
+ * hardware reciprocal square root (about 1 ulp) here; the forward pass, whose
+ * weights must be bit-exact, never goes through this macro. */
+#ifndef SAD_GRAD_SQRT_RSQ
+#define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (sq) = sqrtf(x); (isd) = 1.0f / (sq); } while (0)
+#endif
+
 SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
                            float cr, float cg, float cb, float ux, float uy, float w, float gr, float gg, float gb,
                            float pr, float pg, float pb, float iss, float inv_scale, float c[10])
@@ -352,8 +360,8 @@ SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, 
     float proj2 = proj * proj;
     float perp2 = fmaxf(diff2 - proj2, 0.0f);
     float d2s = fmaxf(fmaf(l1, proj2, l2 * perp2) * iss, 1e-8f);
-    float sq = sqrtf(d2s);
-    float isd = 1.0f / sq;
+    float sq, isd;
+    SAD_GRAD_SQRT_RSQ(d2s, sq, isd);
     float dmix = sq - rn;
     float dl12 = l1 - l2;
     float cpos = dlogit * (tau * iss * isd);
