@@ -123,6 +123,20 @@ int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite
 
 /* ----------------------------------------------------------- optimizer --- */
 
+/* The trainer's per-iteration tail (sad_cuda.cu:2333-2389): the four
+ * tau-diffusion passes (raw = grad_log_tau before the first pass) and Adam
+ * with the gradient clear, the fourth pass and Adam fused into one kernel.
+ * tau_scratch holds 2 * site_count floats.  Bit-identical to sadh_tau_diffuse
+ * x4 (grad_log_tau -> tmp -> grad_log_tau -> tmp -> grad_log_tau) followed by
+ * sadh_adam_update_clear. */
+int sadh_tau_diffuse_adam(const uint32_t* cand0, const uint32_t* cand1, SadSite* sites, SadAdamState* adam,
+                          float* grad_pos_x, float* grad_pos_y, float* grad_log_tau, float* grad_radius,
+                          float* grad_color_r, float* grad_color_g, float* grad_color_b, float* grad_dir_x,
+                          float* grad_dir_y, float* grad_log_aniso, float* tau_scratch, float lambda, int cand_width,
+                          int cand_height, int cand_downscale, float lr_pos, float lr_tau, float lr_radius,
+                          float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps, uint32_t t,
+                          uint32_t site_count, int width, int height, sadh_stream_t stream);
+
 /* launchAdamUpdate, adam.cu:224-251 */
 int sadh_adam_update(SadSite* sites, SadAdamState* adam, const float* grad_pos_x,
                      const float* grad_pos_y, const float* grad_log_tau, const float* grad_radius,

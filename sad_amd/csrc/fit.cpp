@@ -43,6 +43,9 @@ int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const floa
 int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
                 float, uint32_t, uint32_t, int, int, bool, hipStream_t);
 int launch_clear(float* const*, int, uint32_t, hipStream_t);
+int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, float, int, int, int, SadSite*, SadAdamState*,
+                    float* const[10], float, float, float, float, float, float, float, float, float, uint32_t, uint32_t,
+                    int, int, hipStream_t);
 int launch_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const float*, const float*, float*, float*,
                        uint32_t, float, int, int, int, hipStream_t);
 int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float* const[8], uint32_t, const float*,
@@ -379,7 +382,9 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
             SAD_TRY(launch_site_stats(cA0, cA1, d_target, d_mask, d_sites, iss, n, st, width, height, cw, ch, S));
             SAD_TRY(launch_densify_pairs(d_sites, st[0], st[1], pairs, n, 1.0f, o.densify_score_alpha,
                                          (uint32_t)pairs_count, S));
-            SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, (uint32_t)pairs_count, 0xFFFFFFFFu, S));
+            /* pairs at index >= n hold the (0xFFFFFFFF, 0xFFFFFFFF) sentinel: the stable sort of the
+             * whole buffer is the sorted first n pairs followed by them, so only those n are sorted */
+            SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, std::min(n, (uint32_t)pairs_count), 0xFFFFFFFFu, S));
             int k = std::min(desired, std::min(cap - actual, 65536));
             if (k > 0) {
                 SAD_TRY(launch_write_indices(pairs, split_idx, (uint32_t)k, S));
@@ -415,19 +420,15 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         const float blend = (float)iter / (float)std::max(1, o.iterations);
         const float lambda = 0.05f * (0.1f + 0.9f * blend);
-        /* 4 passes: g -> tmp -> g -> tmp -> g; the first also saves the raw gradient */
-        SAD_TRY(launch_tau_diffuse(cA0, cA1, d_sites, g[2], g[2], tau_tmp, tau_raw, n, lambda, cw, ch, ds, S));
-        SAD_TRY(launch_tau_diffuse(cA0, cA1, d_sites, tau_raw, tau_tmp, g[2], nullptr, n, lambda, cw, ch, ds, S));
-        SAD_TRY(launch_tau_diffuse(cA0, cA1, d_sites, tau_raw, g[2], tau_tmp, nullptr, n, lambda, cw, ch, ds, S));
-        SAD_TRY(launch_tau_diffuse(cA0, cA1, d_sites, tau_raw, tau_tmp, g[2], nullptr, n, lambda, cw, ch, ds, S));
-
-        SAD_TRY(launch_adam(d_sites, d_adam, g, lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                            (uint32_t)(iter + 1), n, width, height, true, S));
+        /* 4 tau passes (g -> tmp -> g -> tmp -> g, raw saved by the first), the last fused with Adam + clear */
+        SAD_TRY(launch_tau_adam(cA0, cA1, tau_raw, tau_tmp, lambda, cw, ch, ds, d_sites, d_adam, g, lr_pos, lr_tau,
+                                lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps, (uint32_t)(iter + 1), n, width,
+                                height, S));
 
         if (prune) {
             n_prune++;
             SAD_TRY(launch_prune_pairs(d_sites, removal, pairs, n, delta_norm, (uint32_t)pairs_count, S));
-            SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, (uint32_t)pairs_count, 0xFFFFFFFFu, S));
+            SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, std::min(n, (uint32_t)pairs_count), 0xFFFFFFFFu, S));
             int k = std::min(std::max(0, (int)((float)active_est * p_pct)), 65536);
             if (k > 0) {
                 SAD_TRY(launch_write_indices(pairs, prune_idx, (uint32_t)k, S));

@@ -23,18 +23,9 @@ struct AdamArgs {
     uint32_t n;
 };
 
-template <bool CLEAR>
-__global__ __launch_bounds__(256) void k_adam(AdamArgs A)
+/* one site's update from its normalised gradient g[10] */
+__device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const float g[10])
 {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= A.n) return;
-    float g[10];
-#pragma unroll
-    for (int j = 0; j < 10; ++j) g[j] = A.g[j][i] * A.norm;
-    if (CLEAR) {
-#pragma unroll
-        for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
-    }
     float2* sp = reinterpret_cast<float2*>(A.sites + i);
     float2 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3], s4 = sp[4];
     /* s0 = pos, s1 = (log_tau, radius), s2 = (r, g), s3 = (b, dir.x), s4 = (dir.y, log_aniso) */
@@ -95,6 +86,21 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs A)
     ap[0] = a0; ap[1] = a1; ap[2] = a2; ap[3] = a3; ap[4] = a4;
 }
 
+template <bool CLEAR>
+__global__ __launch_bounds__(256) void k_adam(AdamArgs A)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= A.n) return;
+    float g[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = A.g[j][i] * A.norm;
+    if (CLEAR) {
+#pragma unroll
+        for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
+    }
+    adam_site(A, i, g);
+}
+
 /* clearGradientsKernel / clearBufferKernel, adam.cu:183-219 */
 struct ClearArgs {
     float* g[10];
@@ -146,6 +152,55 @@ __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c
     }
     float diag = 1.0f + lambda * (float)cnt;
     gout[i] = (raw + lambda * sum) / diag;
+}
+
+/* ------------------------------------------- last tau pass fused with Adam --
+ * The trainer's per-iteration tail (sad_cuda.cu:2333-2389) is four Jacobi
+ * passes of tau_diffuse_kernel followed by Adam with the gradient clear.  The
+ * fourth pass and Adam touch the same site, so one kernel does both: the pass
+ * result stays in a register and feeds Adam directly.  The pass's reads of
+ * other sites are `gin` (not written here) and the activity test
+ * pos.x >= 0 of neighbours, which Adam cannot change (active positions are
+ * clamped to [0, W-1], inactive sites are not written).  Arithmetic and order
+ * are those of k_tau_diffuse and k_adam, so the result is bit-identical. */
+__global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                       const float* __restrict__ graw, const float* __restrict__ gin,
+                                                       float lambda, int width, int height, float inv_ds, AdamArgs A)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = A.n;
+    if (i >= n) return;
+    const float2 pos = reinterpret_cast<const float2*>(A.sites + i)[0];
+    float v;
+    if (lambda <= 0.0f || pos.x < 0.0f) {
+        v = gin[i];
+    } else {
+        const float raw = graw[i];
+        int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
+        int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
+        uint4 a = c0[py * width + px], b = c1[py * width + px];
+        uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        float sum = 0.0f;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t nb = ids[k];
+            bool ok = nb < n && nb != i;
+#pragma unroll
+            for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
+            if (ok && A.sites[nb].pos_x >= 0.0f) {
+                sum += gin[nb];
+                cnt += 1u;
+            }
+        }
+        v = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
+    }
+    float g[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i]) * A.norm;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
+    adam_site(A, i, g);
 }
 
 /* ----------------------------------------------------- densify / prune --- */
@@ -352,12 +407,10 @@ __global__ __launch_bounds__(256) void k_count_active(const SadSite* __restrict_
 
 /* ------------------------------------------------------------ launchers -- */
 
-int launch_adam(SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos, float lr_tau, float lr_radius,
-                float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps, uint32_t t,
-                uint32_t n, int width, int height, bool clear, hipStream_t stream)
+static void fill_adam(AdamArgs& A, SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos, float lr_tau,
+                      float lr_radius, float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps,
+                      uint32_t t, uint32_t n, int width, int height)
 {
-    if (n == 0) return SADH_OK;
-    AdamArgs A;
     A.sites = sites; A.st = st;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
     A.lr_pos = lr_pos; A.lr_tau = lr_tau; A.lr_radius = lr_radius; A.lr_color = lr_color;
@@ -368,9 +421,46 @@ int launch_adam(SadSite* sites, SadAdamState* st, float* const g[10], float lr_p
     A.maxx = (float)(width - 1);
     A.maxy = (float)(height - 1);
     A.n = n;
+}
+
+int launch_adam(SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos, float lr_tau, float lr_radius,
+                float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps, uint32_t t,
+                uint32_t n, int width, int height, bool clear, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    AdamArgs A;
+    fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
+              height);
     if (clear) hipLaunchKernelGGL(k_adam<true>, dim3(div_up(n, 256u)), dim3(256), 0, stream, A);
     else hipLaunchKernelGGL(k_adam<false>, dim3(div_up(n, 256u)), dim3(256), 0, stream, A);
     return launch_status("adam_update");
+}
+
+int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, const float* graw,
+                       const float* gin, float* gout, float* raw_copy, uint32_t n, float lambda, int width,
+                       int height, int ds, hipStream_t stream);
+
+/* The trainer's tail: tau passes 1-3 (g2 -> tmp -> g2 -> tmp, raw saved by the
+ * first) and pass 4 fused with Adam + clear. */
+int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* tmp, float lambda, int cand_width,
+                    int cand_height, int ds, SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos,
+                    float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso, float beta1,
+                    float beta2, float eps, uint32_t t, uint32_t n, int width, int height, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    int rc = launch_tau_diffuse(c0, c1, sites, g[2], g[2], tmp, raw, n, lambda, cand_width, cand_height, ds, stream);
+    if (!rc) rc = launch_tau_diffuse(c0, c1, sites, raw, tmp, g[2], nullptr, n, lambda, cand_width, cand_height, ds,
+                                     stream);
+    if (!rc) rc = launch_tau_diffuse(c0, c1, sites, raw, g[2], tmp, nullptr, n, lambda, cand_width, cand_height, ds,
+                                     stream);
+    if (rc) return rc;
+    AdamArgs A;
+    fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
+              height);
+    hipLaunchKernelGGL(k_tau_last_adam, dim3(div_up(n, 256u)), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), raw, tmp, lambda,
+                       cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), A);
+    return launch_status("tau_diffuse_adam");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
@@ -528,6 +618,25 @@ int sadh_clear_buffer(float* buffer, uint32_t count, sadh_stream_t stream)
 {
     SADH_REQUIRE(buffer, "clear_buffer: null pointer");
     return launch_clear(&buffer, 1, count, to_hip(stream));
+}
+
+int sadh_tau_diffuse_adam(const uint32_t* cand0, const uint32_t* cand1, SadSite* sites, SadAdamState* adam,
+                          float* grad_pos_x, float* grad_pos_y, float* grad_log_tau, float* grad_radius,
+                          float* grad_color_r, float* grad_color_g, float* grad_color_b, float* grad_dir_x,
+                          float* grad_dir_y, float* grad_log_aniso, float* tau_scratch, float lambda, int cand_width,
+                          int cand_height, int cand_downscale, float lr_pos, float lr_tau, float lr_radius,
+                          float lr_color, float lr_dir, float lr_aniso, float beta1, float beta2, float eps, uint32_t t,
+                          uint32_t site_count, int width, int height, sadh_stream_t stream)
+{
+    float* const g[10] = {grad_pos_x, grad_pos_y, grad_log_tau, grad_radius, grad_color_r,
+                          grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
+    for (int j = 0; j < 10; ++j) SADH_REQUIRE(g[j], "tau_diffuse_adam: null gradient array");
+    SADH_REQUIRE(cand0 && cand1 && sites && adam && tau_scratch, "tau_diffuse_adam: null pointer");
+    SADH_REQUIRE(cand_width > 0 && cand_height > 0 && width > 0 && height > 0, "tau_diffuse_adam: bad sizes");
+    return launch_tau_adam(cand0, cand1, tau_scratch, tau_scratch + site_count, lambda, cand_width, cand_height,
+                           cand_downscale, sites, adam, g,
+                           lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, site_count,
+                           width, height, to_hip(stream));
 }
 
 int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite* sites, const float* grad_raw,

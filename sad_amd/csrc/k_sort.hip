@@ -39,37 +39,39 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint2* __restrict__ in,
     hist[t * tiles + blockIdx.x] = h[t];
 }
 
-/* Exclusive scan of len words in place by one 1024-thread workgroup. */
+/* Exclusive scan of len words (len a multiple of 4) in place by one
+ * 1024-thread workgroup, in chunks of 4096: each thread loads 4 consecutive
+ * words as one uint4 (coalesced), scans them, and a wave scan plus a scan of
+ * the 16 wave totals gives the chunk's offsets; a carry links the chunks. */
 __global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ data, uint32_t len)
 {
-    __shared__ uint32_t part[1024];
     __shared__ uint32_t wsum[16];
-    uint32_t t = threadIdx.x;
-    uint32_t per = (len + 1023u) / 1024u;
-    uint32_t b = t * per, e = min(b + per, len);
-    uint32_t s = 0u;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    /* inclusive scan of s across the workgroup: wave scan, then wave totals */
-    uint32_t lane = t & 63u, w = t >> 6;
-    uint32_t v = s;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t carry = 0u;
+    for (uint32_t base = 0; base < len; base += 4096u) {
+        const uint32_t i0 = base + 4u * t;
+        uint4 x = make_uint4(0u, 0u, 0u, 0u);
+        if (i0 < len) x = *reinterpret_cast<const uint4*>(data + i0);
+        const uint32_t s1 = x.x + x.y, s2 = s1 + x.z, tot = s2 + x.w;
+        uint32_t v = tot;
 #pragma unroll
-    for (uint32_t off = 1; off < 64u; off <<= 1) {
-        uint32_t o = (uint32_t)__shfl_up((int)v, off);
-        if (lane >= off) v += o;
-    }
-    if (lane == 63u) wsum[w] = v;
-    __syncthreads();
-    if (t < 16u) {
-        uint32_t acc = 0u;
-        for (uint32_t j = 0; j < t; ++j) acc += wsum[j];
-        part[t] = acc;
-    }
-    __syncthreads();
-    uint32_t run = part[w] + v - s;   /* exclusive prefix of this thread's segment */
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t x = data[i];
-        data[i] = run;
-        run += x;
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            uint32_t o = (uint32_t)__shfl_up((int)v, off);
+            if (lane >= off) v += o;
+        }
+        if (lane == 63u) wsum[w] = v;
+        __syncthreads();
+        uint32_t before = 0u, all = 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 16u; ++j) {
+            const uint32_t ws = wsum[j];
+            before += (j < w) ? ws : 0u;
+            all += ws;
+        }
+        const uint32_t ex = carry + before + v - tot;
+        if (i0 < len) *reinterpret_cast<uint4*>(data + i0) = make_uint4(ex, ex + x.x, ex + s1, ex + s2);
+        carry += all;
+        __syncthreads();
     }
 }
 

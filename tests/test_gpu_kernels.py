@@ -289,6 +289,37 @@ def test_adam_parity(gpu, orc, clear):
         assert np.array_equal(gz, grads)
 
 
+@pytest.mark.parametrize("w,h,n,lam", [(160, 120, 1500, 0.037), (300, 200, 70000, 0.05), (64, 64, 300, 0.0)])
+def test_tau_diffuse_adam_fused(gpu, orc, w, h, n, lam):
+    """The fused tail (tau passes, the last one fused with Adam + clear) against the five
+    separate launches: bit-identical sites, state and cleared gradients; the
+    separate chain is itself pinned to the oracle by the tests above."""
+    sites, c0, c1 = _scene(orc, w, h, n, 16, passes=6)
+    rng = np.random.default_rng(31)
+    state = np.abs(rng.normal(0, 1e-3, (n, 20))).astype(np.float32)
+    state[:, 18:] = 0.0
+    grads = rng.normal(0, 50, (10, n)).astype(np.float32)
+    lrs = (0.025, 0.01, 0.01, 0.01, 0.0025, 0.025)
+    g0, g1 = _fields(gpu, c0, c1)
+    # separate: g2 -> tmp -> g2 -> tmp -> g2 with raw = the first input, then Adam with clear
+    s_a, st_a = gpu.put(sites), gpu.put(state)
+    ga = [gpu.put(grads[j]) for j in range(10)]
+    raw, tmp = gpu.put(grads[2]), gpu.zeros((n,))
+    gpu.call("sadh_tau_diffuse", g0, g1, s_a, raw, ga[2], tmp, n, lam, w, h, 1)
+    gpu.call("sadh_tau_diffuse", g0, g1, s_a, raw, tmp, ga[2], n, lam, w, h, 1)
+    gpu.call("sadh_tau_diffuse", g0, g1, s_a, raw, ga[2], tmp, n, lam, w, h, 1)
+    gpu.call("sadh_tau_diffuse", g0, g1, s_a, raw, tmp, ga[2], n, lam, w, h, 1)
+    gpu.call("sadh_adam_update_clear", s_a, st_a, *ga, *lrs, 0.9, 0.999, 1e-8, 7, n, w, h)
+    # fused
+    s_b, st_b = gpu.put(sites), gpu.put(state)
+    gb = [gpu.put(grads[j]) for j in range(10)]
+    gpu.call("sadh_tau_diffuse_adam", g0, g1, s_b, st_b, *gb, gpu.zeros((2 * n,)), lam, w, h, 1, *lrs, 0.9, 0.999, 1e-8,
+             7, n, w, h)
+    assert np.array_equal(gpu.get(s_b), gpu.get(s_a))
+    assert np.array_equal(gpu.get(st_b), gpu.get(st_a))
+    assert all(np.all(gpu.get(g) == 0) for g in gb)
+
+
 def test_score_pairs_and_edits(gpu, orc):
     w, h, n, cap = 256, 256, 3000, 4096
     sites = np.zeros((cap, 10), np.float32)
