@@ -15,6 +15,13 @@ launch of the same kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE passes, scaled
 by a copy kernel of known size (profiles/k_pixel_grad_<workload>_traffic.md).
 cpu_baseline: the CPU oracle (C, OpenMP) on the first iterations of the same
 fit, as a bounded sample.
+
+Other workloads (not the headline line):
+  --workload fit256      the 256x256 / 2,048-site plain fit (100 iterations);
+                         with --cpu-iters 100 the CPU leg is the whole fit.
+  --workload render4096  per-kernel timings at 4096x4096 / 1,048,576 sites
+                         (sad_amd.kernel_bench) with the CPU oracle on the same
+                         inputs; the line carries every kernel's row.
 """
 import argparse
 import json
@@ -46,12 +53,62 @@ def pmc_traffic(workload: str):
     return None
 
 
+def kernel_workload(args) -> int:
+    """--workload render4096: per-kernel timings at 4096x4096 / 1,048,576 sites
+    (sad_amd.kernel_bench), and the CPU oracle on the same inputs."""
+    from sad_amd import kernel_bench
+    inputs: dict = {}
+    rows = kernel_bench.run(keep_inputs=inputs)
+    cpu = None
+    if not args.no_cpu:
+        import numpy as np
+        from oracle import orc
+        threads = min(os.cpu_count() or 1, 16)
+        orc.set_threads(threads)
+        i = inputs
+        w, h, n = i["width"], i["height"], len(i["sites"])
+        timings = {}
+        t0 = time.perf_counter()
+        orc.render(i["cand0"], i["cand1"], i["sites"], i["iss"], w, h)
+        timings["render"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.gradients(i["cand0"], i["cand1"], i["target"], np.ones((h, w), np.float32), i["sites"], i["iss"], w, h)
+        timings["gradient"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.update_candidates(i["cand0"], i["cand1"], orc.pack_sites(i["sites"]), (1 << 16) | 100, i["iss"], w, h,
+                              inject_count=4, hilbert_order=i["hilbert_order"], hilbert_pos=i["hilbert_pos"],
+                              hilbert_probes=1, hilbert_window=32)
+        timings["vpt"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.adam(i["sites"], np.zeros((n, 20), np.float32), np.zeros((10, n), np.float32), i["lrs"], 0.9, 0.999,
+                 1e-8, 1, w, h)
+        timings["adam"] = time.perf_counter() - t0
+        cpu = {"value": round(w * h / 1e6 / timings["gradient"], 3), "unit": "MP/s (one gradient backward)",
+               "cores": threads, "kind": "port",
+               "sample": "one launch each of render, gradient, VPT pass, Adam on the same 4096x4096 inputs",
+               "seconds": {k: round(v, 4) for k, v in timings.items()}}
+    grad = next(r for r in rows if r["kernel"] == "k_pixel<GRAD>")
+    line = {
+        "metric": "gradient backward GB/s (algorithmic) at 4096x4096 / 1,048,576 sites",
+        "value": grad["gb_s"], "unit": "GB/s", "n_gpus": 1, "steps": 20, "warmup": 1,
+        "ms_per_step": round(grad["median_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "render4096", "width": 4096, "height": 4096, "sites": 1 << 20},
+        "roofline": {"bound": "hbm", "achieved": grad["gb_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": grad["frac_hbm"], "traffic": None, "kernel": "k_pixel<MODE_GRAD>"},
+        "kernels": rows,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fit1024")
+    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096"])
     ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -60,7 +117,9 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    import numpy as np
+    if args.workload == "render4096":
+        return kernel_workload(args)
+
     import torch
 
     from sad_amd import config, synth, trainer
