@@ -264,6 +264,28 @@ typedef struct SadhFitResult {
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does
  * (sad_cuda.cu:1871-1905, solveTargetBpp :962-1066, planSiteCapacity :1079-1098). */
+/* One image of a batched fit.  All images share one SadhFitOptions (its
+ * n_sites and init_mode are ignored: each job brings n_sites initial sites). */
+typedef struct SadhBatchJob {
+    const float* target;              /* host [height][width][3] */
+    const float* mask;                /* host [height][width] or NULL */
+    int width, height;
+    SadSite* sites_io;                /* host: n_sites initial sites in; capacity sites out */
+    int n_sites;
+    int sites_cap;                    /* entries of sites_io (>= the job's plan capacity) */
+    float* log_psnr;                  /* optional PSNR log */
+    int log_cap;
+    SadhFitResult* result;            /* optional per-job result */
+} SadhBatchJob;
+
+typedef struct SadhBatchResult {
+    double train_seconds;             /* host wall time of the whole batch's iteration loop */
+    double grad_kernel_ms;            /* summed batched-gradient launch time (profile_gradients) */
+    int grad_launches;
+    double grad_bytes;                /* summed algorithmic bytes of those launches */
+    double total_pixels;              /* sum of W*H over the jobs */
+} SadhBatchResult;
+
 int sadh_fit_plan(const SadhFitOptions* opts, int width, int height, float* densify_percentile,
                   float* prune_percentile, int* capacity, int* pair_count);
 
@@ -275,6 +297,12 @@ int sadh_fit_plan(const SadhFitOptions* opts, int width, int height, float* dens
 int sadh_fit(const SadhFitOptions* opts, const float* target, const float* mask, int width,
              int height, SadSite* sites_io, int sites_cap, float* log_psnr, int log_cap,
              SadhFitResult* result);
+
+/* Batched fit (no reference counterpart: the reference fits one image per run,
+ * sad_cuda.cu:1860-2533).  Every job follows the single-image schedule of
+ * sadh_fit; each iteration's VPT pass, gradient and tau/Adam kernels run as
+ * one launch over all jobs (a device table of per-image descriptors). */
+int sadh_fit_batch(const SadhFitOptions* opts, SadhBatchJob* jobs, int n_jobs, SadhBatchResult* result);
 
 /* Render mode of renderVoronoiFromSites, sad_cuda.cu:1678-1858: JFA (jfa_rounds
  * rounds of clear + seed + log2 floods) and `cand_passes` VPT passes, then the

@@ -49,6 +49,22 @@ class FitResult(C.Structure):
     ]
 
 
+class BatchJob(C.Structure):
+    """SadhBatchJob (include/sad_hip.h)."""
+    _fields_ = [
+        ("target", P), ("mask", P), ("width", I32), ("height", I32), ("sites_io", P), ("n_sites", I32),
+        ("sites_cap", I32), ("log_psnr", P), ("log_cap", I32), ("result", C.POINTER(FitResult)),
+    ]
+
+
+class BatchResult(C.Structure):
+    """SadhBatchResult (include/sad_hip.h)."""
+    _fields_ = [
+        ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
+        ("grad_bytes", C.c_double), ("total_pixels", C.c_double),
+    ]
+
+
 # name -> argument types (all return int)
 SIGNATURES = {
     "sadh_render_voronoi": [P, P, P, P, F32, U32, I32, I32, I32, I32, P],
@@ -81,6 +97,7 @@ SIGNATURES = {
     "sadh_fit_plan": [C.POINTER(FitOptions), I32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32),
                       C.POINTER(I32)],
     "sadh_fit": [C.POINTER(FitOptions), P, P, I32, I32, P, I32, P, I32, C.POINTER(FitResult)],
+    "sadh_fit_batch": [C.POINTER(FitOptions), C.POINTER(BatchJob), I32, C.POINTER(BatchResult)],
     "sadh_render_sites": [P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
                           C.POINTER(F32)],
 }

@@ -1,0 +1,77 @@
+/*
+ * batch.h - device descriptor of one image ("job") of a batched fit.
+ *
+ * The batched fit advances many images in lock step.  Each per-iteration
+ * kernel (VPT search, tiled gradients, tau diffusion, tau + Adam) is one launch
+ * over all images: its grid is the concatenation of every job's 16x16 tiles or
+ * sites, and a block finds its job by binary search over the jobs' first tile
+ * (or site) index.  The descriptors live in device memory; the host changes
+ * only `n` (after densify) and `cur` (after a candidate update) with a small
+ * kernel, in stream order.
+ */
+#ifndef SAD_BATCH_H
+#define SAD_BATCH_H
+
+#include "sad_device.h"
+
+namespace sadh {
+
+struct BatchJob {
+    /* image */
+    const float* target;
+    const float* mask;
+    int width, height;         /* pixels */
+    int cw, ch, ds;            /* candidate grid and downscale */
+    float iss, inv_scale;      /* 1/max(W,H)^2 and its square root */
+    float norm, maxx, maxy;    /* Adam: 1/(W*H), W-1, H-1 */
+    /* candidate fields: two plane pairs, ping-ponged by the VPT pass */
+    uint32_t* fa0;
+    uint32_t* fa1;
+    uint32_t* fb0;
+    uint32_t* fb1;
+    /* sites and their per-site buffers (capacity entries each) */
+    SadSite* sites;
+    SadAdamState* adam;
+    SadSearchSite* search;
+    float* g[10];
+    float* tau_raw;
+    float* tau_tmp;
+    float* removal;            /* prune iterations: removal deltas */
+    const uint32_t* h_order;   /* null: no Hilbert probes */
+    const uint32_t* h_pos;
+    uint32_t n;                /* current site count */
+    uint32_t cur;              /* current candidate field: 0 = (fa0, fa1), 1 = (fb0, fb1) */
+    uint32_t cap;              /* site capacity (flattened site range) */
+    /* offsets of this job in the flattened grids */
+    uint32_t tiles_x, ntiles, tile_begin;     /* 16x16 pixel tiles (gradient) */
+    uint32_t ctiles_x, cntiles, ctile_begin;  /* 16x16 candidate-cell tiles (VPT) */
+    uint32_t site_begin;
+};
+
+/* job owning flattened index f, given each job's first index `begin` (ascending) */
+__device__ __forceinline__ uint32_t batch_find(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t f,
+                                               uint32_t BatchJob::*begin)
+{
+    uint32_t lo = 0u, hi = njobs;   /* jobs[lo].begin <= f < jobs[hi].begin */
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].*begin <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* iteration scalars shared by every job of a batched launch */
+struct BatchStep {
+    float lambda;              /* tau diffusion weight */
+    float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
+    float beta1, beta2, eps, b1c, b2c;
+    uint32_t jump_index;       /* VPT pass index (the jump step depends on each job's grid) */
+    uint32_t vpt_parity;       /* VPT pass p of an update reads field cur ^ (p & 1), writes the other */
+    float radius_scale;
+    uint32_t radius_probes, inject_count, hil_probes, hil_window;
+};
+
+}  // namespace sadh
+
+#endif /* SAD_BATCH_H */

@@ -1,0 +1,108 @@
+/*
+ * fit_common.h - host-side pieces shared by the single-image and batched fit
+ * drivers: kernel launcher declarations, status macros, RAII device arena and
+ * stream pair, and the capacity/percentile plan.
+ */
+#ifndef SAD_FIT_COMMON_H
+#define SAD_FIT_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+
+namespace sadh {
+
+int launch_render(const uint32_t*, const uint32_t*, float*, const SadSite*, float, uint32_t, int, int, int, int,
+                  hipStream_t);
+int launch_gradients(const uint32_t*, const uint32_t*, const float*, const float*, float* const[10], const SadSite*,
+                     float, uint32_t, float*, bool, int, int, int, int, hipStream_t);
+int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
+                      float* const[8], int, int, int, int, hipStream_t);
+int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
+                float, uint32_t, uint32_t, int, int, bool, hipStream_t);
+int launch_clear(float* const*, int, uint32_t, hipStream_t);
+int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, float, int, int, int, SadSite*, SadAdamState*,
+                    float* const[10], float, float, float, float, float, float, float, float, float, uint32_t, uint32_t,
+                    int, int, hipStream_t);
+int launch_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const float*, const float*, float*, float*,
+                       uint32_t, float, int, int, int, hipStream_t);
+int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float* const[8], uint32_t, const float*,
+                 int, int, hipStream_t);
+int launch_sse(const float*, const float*, const float*, float*, int, int, hipStream_t);
+int launch_count_active(const SadSite*, uint32_t*, uint32_t, hipStream_t);
+int launch_densify_pairs(const SadSite*, const float*, const float*, SadPair*, uint32_t, float, float, uint32_t,
+                         hipStream_t);
+int launch_prune_pairs(const SadSite*, const float*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
+int launch_write_indices(const SadPair*, uint32_t*, uint32_t, hipStream_t);
+int launch_prune_sites(SadSite*, const uint32_t*, uint32_t, hipStream_t);
+int launch_hilbert_pairs(const SadSite*, SadPair*, uint32_t, uint32_t, int, int, int, hipStream_t);
+int launch_hilbert_order(const SadPair*, uint32_t*, uint32_t*, uint32_t, hipStream_t);
+int launch_radix_sort(SadPair*, SadPair*, uint32_t*, uint32_t, uint32_t, hipStream_t);
+int launch_vpt_search(const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, const SadSearchSite*, uint32_t,
+                      uint32_t, float, uint32_t, float, uint32_t, uint32_t, const uint32_t*, const uint32_t*, uint32_t,
+                      uint32_t, int, int, int, int, int, hipStream_t);
+int launch_search_sites(const SadSite*, SadSearchSite*, uint32_t, float, bool, hipStream_t);
+int launch_jfa_flood_search(const uint32_t*, uint32_t*, const SadSearchSite*, uint32_t, uint32_t, float, int, int,
+                            int, int, int, hipStream_t);
+
+/* ------------------------------------------------------------ helpers --- */
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) return fail((int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define SAD_TRY(expr)                  \
+    do {                               \
+        int r_ = (expr);               \
+        if (r_ != SADH_OK) return r_;  \
+    } while (0)
+
+/* Device allocations released at scope exit. */
+struct DevArena {
+    std::vector<void*> ptrs;
+    ~DevArena()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T** out, size_t count)
+    {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return fail((int)e, "hipMalloc(" + std::to_string(bytes) + " B) failed");
+        ptrs.push_back(p);
+        *out = static_cast<T*>(p);
+        return SADH_OK;
+    }
+};
+
+struct StreamPair {
+    hipStream_t main = nullptr, cand = nullptr;
+    hipEvent_t cand_done = nullptr, sites_ready = nullptr;
+    ~StreamPair()
+    {
+        if (cand_done) (void)hipEventDestroy(cand_done);
+        if (sites_ready) (void)hipEventDestroy(sites_ready);
+        if (cand) (void)hipStreamDestroy(cand);
+        if (main) (void)hipStreamDestroy(main);
+    }
+};
+
+/* batched fit (one launch over every job) */
+int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);
+int launch_vpt_batch(const BatchJob*, uint32_t, uint32_t, uint32_t, const BatchStep&, hipStream_t);
+int launch_tau_adam_batch(const BatchJob*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
+int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
+
+/* capacity and densify/prune percentiles of a fit (sad_cuda.cu:904-1098) */
+int plan(const SadhFitOptions& o, int width, int height, float* d_pct, float* p_pct, int* cap, int* pairs);
+
+}  // namespace sadh
+
+#endif /* SAD_FIT_COMMON_H */

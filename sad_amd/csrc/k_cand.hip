@@ -14,7 +14,7 @@
  * key (tau, e^a, e^-a, r/S) already evaluated, which saves two exp, one division
  * and the fp16 decodes per candidate and gives identical bits.
  */
-#include "sad_device.h"
+#include "batch.h"
 
 namespace sadh {
 
@@ -245,20 +245,18 @@ __global__ __launch_bounds__(256) void k_jfa_flood(const uint4* __restrict__ in0
 
 /* ------------------------------------------------------------- VPT pass --
  * updateCandidatesKernel, jfa.cu:271-386 */
+/* One candidate cell (x, y) of a VPT pass. */
 template <class Fetch>
-__global__ __launch_bounds__(256) void k_vpt(const uint4* __restrict__ in0, const uint4* __restrict__ in1,
-                                             uint4* __restrict__ out0, uint4* __restrict__ out1,
-                                             Fetch fetch, uint32_t n, uint32_t step, float iss,
-                                             uint32_t step_high, float radius_scale,
-                                             uint32_t radius_probes, uint32_t inject_count,
-                                             const uint32_t* __restrict__ hil_order,
-                                             const uint32_t* __restrict__ hil_pos,
-                                             uint32_t hil_probes, uint32_t hil_window, int ds,
-                                             int tw, int th, int width, int height)
+__device__ __forceinline__ void vpt_cell(int x, int y, const uint4* __restrict__ in0, const uint4* __restrict__ in1,
+                                         uint4* __restrict__ out0, uint4* __restrict__ out1,
+                                         const Fetch& fetch, uint32_t n, uint32_t step, float iss,
+                                         uint32_t step_high, float radius_scale,
+                                         uint32_t radius_probes, uint32_t inject_count,
+                                         const uint32_t* __restrict__ hil_order,
+                                         const uint32_t* __restrict__ hil_pos,
+                                         uint32_t hil_probes, uint32_t hil_window, int ds,
+                                         int tw, int th, int width, int height)
 {
-    int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= width || y >= height) return;
     uint32_t step_index = step & 0xFFFFu;
     uint32_t jump = step >> 16;
     if (jump == 0u) jump = 1u;
@@ -328,6 +326,65 @@ __global__ __launch_bounds__(256) void k_vpt(const uint4* __restrict__ in0, cons
     out1[y * width + x] = make_uint4(bi[4], bi[5], bi[6], bi[7]);
 }
 
+template <class Fetch>
+__global__ __launch_bounds__(256) void k_vpt(const uint4* __restrict__ in0, const uint4* __restrict__ in1,
+                                             uint4* __restrict__ out0, uint4* __restrict__ out1,
+                                             Fetch fetch, uint32_t n, uint32_t step, float iss,
+                                             uint32_t step_high, float radius_scale,
+                                             uint32_t radius_probes, uint32_t inject_count,
+                                             const uint32_t* __restrict__ hil_order,
+                                             const uint32_t* __restrict__ hil_pos,
+                                             uint32_t hil_probes, uint32_t hil_window, int ds,
+                                             int tw, int th, int width, int height)
+{
+    int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= width || y >= height) return;
+    vpt_cell(x, y, in0, in1, out0, out1, fetch, n, step, iss, step_high, radius_scale, radius_probes, inject_count,
+             hil_order, hil_pos, hil_probes, hil_window, ds, tw, th, width, height);
+}
+
+/* Batched fit: one VPT pass over the concatenated candidate tiles of every job
+ * (each job's current field -> its other field); the jump step comes from each
+ * job's grid (packJumpStep, sad_cuda.cu:1157-1183). */
+__global__ __launch_bounds__(256) void k_vpt_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, BatchStep st)
+{
+    const uint32_t f = blockIdx.x;
+    const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::ctile_begin);
+    const BatchJob& J = jobs[j];
+    const uint32_t t = f - J.ctile_begin;
+    const int x = (int)(t % J.ctiles_x) * 16 + (int)(threadIdx.x & 15u);
+    const int y = (int)(t / J.ctiles_x) * 16 + (int)(threadIdx.x >> 4);
+    if (x >= J.cw || y >= J.ch) return;
+    const uint32_t src = J.cur ^ st.vpt_parity;
+    const uint4* in0 = reinterpret_cast<const uint4*>(src ? J.fb0 : J.fa0);
+    const uint4* in1 = reinterpret_cast<const uint4*>(src ? J.fb1 : J.fa1);
+    uint4* out0 = reinterpret_cast<uint4*>(src ? J.fa0 : J.fb0);
+    uint4* out1 = reinterpret_cast<uint4*>(src ? J.fa1 : J.fb1);
+    SearchFetch fetch{reinterpret_cast<const float4*>(J.search)};
+    const bool hil = J.h_order != nullptr && st.hil_probes > 0u && st.hil_window > 0u;
+    vpt_cell(x, y, in0, in1, out0, out1, fetch, J.n, sad_pack_jump_step(st.jump_index, J.cw, J.ch), J.iss,
+             st.jump_index >> 16, st.radius_scale, st.radius_probes, st.inject_count, J.h_order, J.h_pos,
+             hil ? st.hil_probes : 0u, hil ? st.hil_window : 0u, J.ds, J.width, J.height, J.cw, J.ch);
+}
+
+/* Batched fit: search sites (fp16-rounded key terms) of every job's sites. */
+__global__ __launch_bounds__(256) void k_search_batch(const BatchJob* __restrict__ jobs, uint32_t njobs)
+{
+    const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::site_begin);
+    const BatchJob& J = jobs[j];
+    const uint32_t i = f - J.site_begin;
+    if (i >= J.n) return;
+    SadSite s = J.sites[i];
+    SadPackedSite p = sad_pack_site(&s);
+    SadSearchSite q = sad_search_site(h2f(p.h[0]), h2f(p.h[1]), h2f(p.h[2]), h2f(p.h[3]),
+                                      h2f(p.h[4]), h2f(p.h[5]), h2f(p.h[6]), J.inv_scale);
+    float4* o = reinterpret_cast<float4*>(J.search + i);
+    o[0] = make_float4(q.px, q.py, q.dx, q.dy);
+    o[1] = make_float4(q.l1, q.l2, q.rn, q.tau);
+}
+
 /* ------------------------------------------------------------- launchers -- */
 
 static inline dim3 grid16(int w, int h) { return dim3(div_up((uint32_t)w, 16u), div_up((uint32_t)h, 16u)); }
@@ -346,6 +403,15 @@ int launch_vpt_search(const uint32_t* in0, const uint32_t* in1, uint32_t* out0, 
                        step_high, radius_scale, radius_probes, inject_count, hil_order, hil_pos,
                        hil_probes, hil_window, ds, tw, th, width, height);
     return launch_status("update_candidates(search)");
+}
+
+int launch_vpt_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_ctiles, uint32_t total_sites,
+                     const BatchStep& st, hipStream_t stream)
+{
+    if (total_sites > 0)
+        hipLaunchKernelGGL(k_search_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs);
+    if (total_ctiles > 0) hipLaunchKernelGGL(k_vpt_batch, dim3(total_ctiles), dim3(256), 0, stream, jobs, njobs, st);
+    return launch_status("update_candidates_batch");
 }
 
 int launch_search_sites(const SadSite* sites, SadSearchSite* out, uint32_t n, float iss, bool from_f16,

@@ -48,7 +48,7 @@
 #ifndef SAD_EXACT_GRAD_MATH
 #define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (isd) = __builtin_amdgcn_rsqf(x); (sq) = (x) * (isd); } while (0)
 #endif
-#include "sad_device.h"
+#include "batch.h"
 
 namespace sadh {
 
@@ -173,14 +173,16 @@ struct PixelArgs {
 };
 
 template <int MODE, bool REMOVAL, int H>
-__global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
+using PixelLds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>,
+                                           TileLds<H, REMOVAL, NumValues<MODE, REMOVAL>::value>>::type;
+
+/* One 16x16 tile (index `tile` of A's image) of any mode; 256 threads, one pixel each. */
+template <int MODE, bool REMOVAL, int H>
+__device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t tile, PixelLds<MODE, REMOVAL, H>& L)
 {
     constexpr int NV = NumValues<MODE, REMOVAL>::value;
     static_assert(H <= 128, "per-pixel slot table holds 7-bit slots");
-    using Lds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>, TileLds<H, REMOVAL, NV>>::type;
-    __shared__ Lds L;
     const uint32_t tid = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, A.ntiles);
     const int tx0 = (int)(tile % A.tiles_x) * 16, ty0 = (int)(tile / A.tiles_x) * 16;
     const int x = tx0 + (int)(tid & 15u);
     const int y = ty0 + (int)(tid >> 4);
@@ -479,6 +481,43 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
     }  /* backward modes */
 }
 
+template <int MODE, bool REMOVAL, int H>
+__global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
+{
+    __shared__ PixelLds<MODE, REMOVAL, H> L;
+    pixel_tile<MODE, REMOVAL, H>(A, xcd_tile(blockIdx.x, A.ntiles), L);
+}
+
+/* Batched fit: one launch over the concatenated tiles of every job. */
+template <bool REMOVAL, int H>
+__global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                          uint32_t total_tiles)
+{
+    __shared__ PixelLds<MODE_GRAD, REMOVAL, H> L;
+    const uint32_t f = xcd_tile(blockIdx.x, total_tiles);
+    const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::tile_begin);
+    const BatchJob& J = jobs[j];
+    PixelArgs A = {};
+    A.cand0 = J.cur ? J.fb0 : J.fa0;
+    A.cand1 = J.cur ? J.fb1 : J.fa1;
+    A.sites = J.sites;
+    A.target = J.target;
+    A.mask = J.mask;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) A.g[k] = J.g[k];
+    A.removal = J.removal;
+    A.iss = J.iss;
+    A.inv_scale = J.inv_scale;
+    A.n = J.n;
+    A.width = J.width;
+    A.height = J.height;
+    A.cw = J.cw;
+    A.ch = J.ch;
+    A.tiles_x = J.tiles_x;
+    A.ntiles = J.ntiles;
+    pixel_tile<MODE_GRAD, REMOVAL, H>(A, f - J.tile_begin, L);
+}
+
 /* ------------------------------------------------------------ launchers -- */
 
 #ifndef SAD_HASH_SLOTS
@@ -516,6 +555,19 @@ int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* 
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     if (compute_removal) return launch_pixel<MODE_GRAD, true>(A, stream, "compute_gradients_tiled");
     return launch_pixel<MODE_GRAD, false>(A, stream, "compute_gradients_tiled");
+}
+
+int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
+                           hipStream_t stream)
+{
+    if (total_tiles == 0) return SADH_OK;
+    if (removal)
+        hipLaunchKernelGGL((k_pixel_grad_batch<true, HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs, njobs,
+                           total_tiles);
+    else
+        hipLaunchKernelGGL((k_pixel_grad_batch<false, HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs,
+                           njobs, total_tiles);
+    return launch_status("compute_gradients_batch");
 }
 
 int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,

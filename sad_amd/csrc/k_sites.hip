@@ -6,7 +6,9 @@
  * 256-thread blocks.  Sites and Adam state are AoS (40 B / 80 B per site, read
  * and written whole with 8- and 16-byte accesses); gradients are SoA.
  */
-#include "sad_device.h"
+#include <algorithm>
+
+#include "batch.h"
 
 namespace sadh {
 
@@ -120,14 +122,12 @@ __global__ __launch_bounds__(256) void k_clear(ClearArgs A)
  * active neighbours found in the candidate list of the site's home cell.
  * raw_copy (nullable) receives grad_raw[i], folding the copy the trainer makes
  * before the first pass (sad_cuda.cu:2335-2337) into that pass. */
-__global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                     const SadSite* __restrict__ sites,
-                                                     const float* __restrict__ graw, const float* __restrict__ gin,
-                                                     float* __restrict__ gout, float* __restrict__ raw_copy,
-                                                     uint32_t n, float lambda, int width, int height, float inv_ds)
+__device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                         const SadSite* __restrict__ sites, const float* __restrict__ graw,
+                                         const float* __restrict__ gin, float* __restrict__ gout,
+                                         float* __restrict__ raw_copy, uint32_t n, float lambda, int width,
+                                         int height, float inv_ds)
 {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
     float raw = graw[i];
     if (raw_copy) raw_copy[i] = raw;
     const float2* sp = reinterpret_cast<const float2*>(sites + i);
@@ -154,6 +154,16 @@ __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c
     gout[i] = (raw + lambda * sum) / diag;
 }
 
+__global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                     const SadSite* __restrict__ sites,
+                                                     const float* __restrict__ graw, const float* __restrict__ gin,
+                                                     float* __restrict__ gout, float* __restrict__ raw_copy,
+                                                     uint32_t n, float lambda, int width, int height, float inv_ds)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) tau_site(i, c0, c1, sites, graw, gin, gout, raw_copy, n, lambda, width, height, inv_ds);
+}
+
 /* ------------------------------------------- last tau pass fused with Adam --
  * The trainer's per-iteration tail (sad_cuda.cu:2333-2389) is four Jacobi
  * passes of tau_diffuse_kernel followed by Adam with the gradient clear.  The
@@ -163,13 +173,12 @@ __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c
  * pos.x >= 0 of neighbours, which Adam cannot change (active positions are
  * clamped to [0, W-1], inactive sites are not written).  Arithmetic and order
  * are those of k_tau_diffuse and k_adam, so the result is bit-identical. */
-__global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                       const float* __restrict__ graw, const float* __restrict__ gin,
-                                                       float lambda, int width, int height, float inv_ds, AdamArgs A)
+__device__ __forceinline__ void tau_last_adam_site(uint32_t i, const uint4* __restrict__ c0,
+                                                   const uint4* __restrict__ c1, const float* __restrict__ graw,
+                                                   const float* __restrict__ gin, float lambda, int width, int height,
+                                                   float inv_ds, const AdamArgs& A)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const uint32_t n = A.n;
-    if (i >= n) return;
     const float2 pos = reinterpret_cast<const float2*>(A.sites + i)[0];
     float v;
     if (lambda <= 0.0f || pos.x < 0.0f) {
@@ -201,6 +210,64 @@ __global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__
 #pragma unroll
     for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
     adam_site(A, i, g);
+}
+
+__global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                       const float* __restrict__ graw, const float* __restrict__ gin,
+                                                       float lambda, int width, int height, float inv_ds, AdamArgs A)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < A.n) tau_last_adam_site(i, c0, c1, graw, gin, lambda, width, height, inv_ds, A);
+}
+
+/* Batched fit: tau pass `pass` (0-2: g2 -> tmp -> g2 -> tmp, raw saved by pass
+ * 0) or, for pass 3, the last pass fused with Adam + clear, over the
+ * concatenated sites of every job; the candidate field is the current one. */
+__global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t pass,
+                                                   BatchStep st)
+{
+    const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t jx = batch_find(jobs, njobs, f, &BatchJob::site_begin);
+    const BatchJob& J = jobs[jx];
+    const uint32_t i = f - J.site_begin;
+    if (i >= J.n) return;
+    const uint4* c0 = reinterpret_cast<const uint4*>(J.cur ? J.fb0 : J.fa0);
+    const uint4* c1 = reinterpret_cast<const uint4*>(J.cur ? J.fb1 : J.fa1);
+    const float inv_ds = 1.0f / fmaxf((float)J.ds, 1.0f);
+    float* g2 = J.g[2];
+    if (pass == 0u) {
+        tau_site(i, c0, c1, J.sites, g2, g2, J.tau_tmp, J.tau_raw, J.n, st.lambda, J.cw, J.ch, inv_ds);
+    } else if (pass == 1u) {
+        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, g2, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+    } else if (pass == 2u) {
+        tau_site(i, c0, c1, J.sites, J.tau_raw, g2, J.tau_tmp, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+    } else {
+        AdamArgs A;
+        A.sites = J.sites; A.st = J.adam;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) A.g[k] = J.g[k];
+        A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
+        A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
+        A.b1c = st.b1c; A.b2c = st.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
+        tau_last_adam_site(i, c0, c1, J.tau_raw, J.tau_tmp, st.lambda, J.cw, J.ch, inv_ds, A);
+    }
+}
+
+/* Batched fit: set n and/or flip the current field of listed jobs, in stream order. */
+struct JobUpdate {
+    uint32_t count;
+    uint32_t job[64];
+    uint32_t n[64];            /* 0xFFFFFFFF: keep */
+    uint32_t flip[64];         /* 1: swap the current candidate field */
+};
+
+__global__ void k_update_jobs(BatchJob* jobs, JobUpdate u)
+{
+    const uint32_t t = threadIdx.x;
+    if (t >= u.count) return;
+    BatchJob& J = jobs[u.job[t]];
+    if (u.n[t] != 0xFFFFFFFFu) J.n = u.n[t];
+    if (u.flip[t]) J.cur ^= 1u;
 }
 
 /* ----------------------------------------------------- densify / prune --- */
@@ -461,6 +528,32 @@ int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* t
                        reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), raw, tmp, lambda,
                        cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), A);
     return launch_status("tau_diffuse_adam");
+}
+
+int launch_tau_adam_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_sites, const BatchStep& st,
+                          hipStream_t stream)
+{
+    if (total_sites == 0) return SADH_OK;
+    for (uint32_t pass = 0; pass < 4u; ++pass)
+        hipLaunchKernelGGL(k_tau_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs, pass, st);
+    return launch_status("tau_diffuse_adam_batch");
+}
+
+/* job_index[k] gets n[k] (0xFFFFFFFF keeps it) and flips its field when flip[k] */
+int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t* n, const uint32_t* flip, uint32_t k,
+                       hipStream_t stream)
+{
+    for (uint32_t b = 0; b < k; b += 64u) {
+        JobUpdate u = {};
+        u.count = std::min(64u, k - b);
+        for (uint32_t t = 0; t < u.count; ++t) {
+            u.job[t] = job_index[b + t];
+            u.n[t] = n[b + t];
+            u.flip[t] = flip[b + t];
+        }
+        hipLaunchKernelGGL(k_update_jobs, dim3(1), dim3(64), 0, stream, jobs, u);
+    }
+    return launch_status("batch_update_jobs");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)

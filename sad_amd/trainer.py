@@ -53,6 +53,56 @@ def fit(target: np.ndarray, sites: np.ndarray, opts: dict | None = None, mask: n
     return out
 
 
+def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list | None = None, **over) -> dict:
+    """Fit many images in lock step with one launch per kernel per iteration
+    (sadh_fit_batch).  targets[i] float32 [H][W][3], sites[i] float32 [N_i][10];
+    all jobs share the options.  Returns {"jobs": [per-job dicts as fit()],
+    "train_seconds", "grad_kernel_ms", "grad_launches", "grad_bytes", "total_pixels"}."""
+    o = dict(opts or fit_options())
+    o.update(over)
+    if o["init_mode"] != 0:
+        raise ValueError("fit_batch takes caller-provided initial sites (init_mode 0)")
+    n = len(targets)
+    if n == 0 or len(sites) != n or (masks is not None and len(masks) != n):
+        raise ValueError("targets, sites (and masks) must be non-empty lists of equal length")
+    keep = []
+    jobs = (_lib.BatchJob * n)()
+    results = [_lib.FitResult() for _ in range(n)]
+    outs = []
+    for i, (tgt, st) in enumerate(zip(targets, sites)):
+        h, w, _ = tgt.shape
+        oi = dict(o, n_sites=int(st.shape[0]))
+        plan = fit_plan(oi, w, h)
+        cap = max(plan["capacity"], st.shape[0])
+        buf = np.zeros((cap, 10), np.float32)
+        buf[:, 0:2] = -1.0
+        buf[: st.shape[0]] = st
+        t = np.ascontiguousarray(tgt, np.float32)
+        m = None if masks is None or masks[i] is None else np.ascontiguousarray(masks[i], np.float32)
+        logs = np.zeros(64, np.float32)
+        keep += [t, m, buf, logs]
+        jobs[i].target = _lib.ptr(t)
+        jobs[i].mask = _lib.ptr(m)
+        jobs[i].width, jobs[i].height = w, h
+        jobs[i].sites_io = _lib.ptr(buf)
+        jobs[i].n_sites = int(st.shape[0])
+        jobs[i].sites_cap = cap
+        jobs[i].log_psnr = _lib.ptr(logs)
+        jobs[i].log_cap = len(logs)
+        jobs[i].result = C.pointer(results[i])
+        outs.append(dict(sites=buf, plan=plan, _logs=logs))
+    br = _lib.BatchResult()
+    s = to_struct(o)
+    _lib.call("sadh_fit_batch", C.byref(s), jobs, n, C.byref(br))
+    for out, res in zip(outs, results):
+        out.update({k: getattr(res, k) for k, _ in _lib.FitResult._fields_ if not k.startswith("grad_ms_trace")})
+        logs = out.pop("_logs")
+        out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
+    ret = {k: getattr(br, k) for k, _ in _lib.BatchResult._fields_}
+    ret["jobs"] = outs
+    return ret
+
+
 def render_sites(sites: np.ndarray, width: int, height: int, cand_passes: int = 16, use_jfa: bool = True,
                  jfa_rounds: int = 1, cand_downscale: int = 1, radius_scale: float = 64.0, radius_probes: int = 0,
                  inject_count: int = 4, hilbert_window: int = 32, hilbert_probes: int = 1) -> dict:

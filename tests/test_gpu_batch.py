@@ -1,0 +1,66 @@
+"""Batched fit (sadh_fit_batch): every job of a batch must fit as it does
+alone, and as the oracle's restatement of the single-image schedule does.
+
+Bars: the host schedule (densify/prune counts, final site counts, active-site
+estimates, capacities) identical to the oracle's; final PSNR of every job
+within 0.05 dB of the oracle's fit of that image alone."""
+import numpy as np
+import pytest
+
+from sad_amd import config, synth, trainer
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(specs):
+    tg, st = [], []
+    for (w, h, seed, n) in specs:
+        t = synth.make_target(w, h, seed, n_ellipses=20)
+        tg.append(t)
+        st.append(synth.init_sites_uniform(t, n, seed))
+    return tg, st
+
+
+def test_batch_of_one_matches_single_fit(orc):
+    tg, st = _jobs([(96, 80, 51, 300)])
+    opts = config.fit_options(iterations=80, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
+                              prune_percentile=0.0, n_sites=300, max_sites=300, log_freq=40)
+    b = trainer.fit_batch(tg, st, opts)["jobs"][0]
+    g = trainer.fit(tg[0], st[0], opts)
+    r = orc.fit(tg[0], st[0], opts)
+    print("batch", b["log_psnr"], "single", g["log_psnr"], "oracle", r["log_psnr"])
+    assert b["n_cand_updates"] == g["n_cand_updates"] == r["n_cand_updates"]
+    assert abs(b["final_psnr"] - g["final_psnr"]) <= 0.05
+    assert abs(b["final_psnr"] - r["final_psnr"]) <= 0.05
+
+
+def test_batch_mixed_sizes_densify_prune(orc):
+    """Three images of different sizes and aspect ratios with densify and
+    prune every 20 iterations: per-job schedule and PSNR against the oracle."""
+    specs = [(64, 48, 61, 150), (150, 100, 62, 500), (90, 170, 63, 400)]
+    tg, st = _jobs(specs)
+    opts = config.fit_options(iterations=120, target_bpp=1.5, densify_freq=20, prune_freq=20, densify_start=20,
+                              prune_start=40, max_sites=2000, log_freq=60)
+    out = trainer.fit_batch(tg, st, opts)
+    assert out["total_pixels"] == sum(w * h for w, h, _, _ in specs)
+    for i, (t, s0) in enumerate(zip(tg, st)):
+        b = out["jobs"][i]
+        r = orc.fit(t, s0, dict(opts, n_sites=s0.shape[0]))
+        print(i, "batch", b["log_psnr"], b["actual_sites"], "oracle", r["log_psnr"], r["actual_sites"])
+        assert b["plan"] == r["plan"]
+        assert (b["n_densify"], b["n_prune"], b["actual_sites"], b["active_estimate"]) == \
+               (r["n_densify"], r["n_prune"], r["actual_sites"], r["active_estimate"])
+        assert b["active_count"] == int(np.sum(b["sites"][:, 0] >= 0))
+        assert abs(b["final_psnr"] - r["final_psnr"]) <= 0.05
+
+
+def test_batch_rejects_bad_jobs():
+    from sad_amd._lib import SadHipError
+    tg, st = _jobs([(64, 48, 71, 100)])
+    with pytest.raises(ValueError):
+        trainer.fit_batch([], [], config.fit_options())
+    opts = config.fit_options(iterations=4, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
+                              prune_percentile=0.0)
+    bad = synth.sites_array(0)
+    with pytest.raises(SadHipError):
+        trainer.fit_batch(tg, [bad], opts)
