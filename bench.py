@@ -22,6 +22,8 @@ Other workloads (not the headline line):
   --workload render4096  per-kernel timings at 4096x4096 / 1,048,576 sites
                          (sad_amd.kernel_bench) with the CPU oracle on the same
                          inputs; the line carries every kernel's row.
+  --workload batch64     64 images (about 60 MP) fitted as one batch for 500
+                         iterations, and one at a time on the same inputs.
 """
 import argparse
 import json
@@ -103,12 +105,74 @@ def kernel_workload(args) -> int:
     return 0
 
 
+def batch_workload(args) -> int:
+    """--workload batch64: 64 images (sides log-uniform in [128, 2048], aspect
+    log-uniform in [0.5, 2], bpp 1.0 initial sites), 500 iterations, fitted as
+    one batch (sadh_fit_batch) and, for comparison, one image at a time
+    (sadh_fit) on the same inputs."""
+    import numpy as np
+    import torch
+
+    from sad_amd import config, synth, trainer
+    torch.cuda.set_device(0)
+    targets, sites = synth.batch_workload()
+    iters = 500
+    opts = config.fit_options(iterations=iters, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
+                              prune_percentile=0.0, log_freq=10**9, profile_gradients=1)
+    mp = sum(t.shape[0] * t.shape[1] for t in targets) / 1e6
+    for _ in range(args.warmup):
+        trainer.fit_batch(targets[:4], sites[:4], opts)
+    torch.cuda.synchronize()
+    runs = [trainer.fit_batch(targets, sites, opts) for _ in range(args.steps)]
+    loop_s = sum(r["train_seconds"] for r in runs)
+    value = mp * iters * args.steps / loop_s
+    last = runs[-1]
+    single_s, dpsnr = 0.0, 0.0
+    for i, (t, s0) in enumerate(zip(targets, sites)):
+        r = trainer.fit(t, s0, opts)
+        single_s += r["train_seconds"]
+        dpsnr = max(dpsnr, abs(r["final_psnr"] - last["jobs"][i]["final_psnr"]))
+    single = mp * iters / single_s
+    gms, gb = last["grad_kernel_ms"], last["grad_bytes"]
+    achieved = gb / (gms * 1e-3) / 1e9 if gms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        threads = min(os.cpu_count() or 1, 16)
+        orc.set_threads(threads)
+        k, ci = 4, max(1, args.cpu_iters // 6)
+        c0 = time.perf_counter()
+        for t, s0 in zip(targets[:k], sites[:k]):
+            orc.fit(t, s0, dict(opts, iterations=ci, n_sites=s0.shape[0], profile_gradients=0))
+        cs = time.perf_counter() - c0
+        cmp = sum(t.shape[0] * t.shape[1] for t in targets[:k]) / 1e6
+        cpu = {"value": cmp * ci / cs, "unit": "MP-iters/s", "cores": threads, "kind": "port",
+               "sample": f"first {ci} iterations of the first {k} images ({cmp:.2f} MP, {cs:.1f} s, OpenMP oracle)"}
+    line = {
+        "metric": "fit megapixel-iters/sec (fwd+bwd+Adam)", "value": round(value, 3), "unit": "MP-iters/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * loop_s / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch64", "images": len(targets), "megapixels": round(mp, 3), "iterations": iters,
+                   "sites": int(sum(len(s0) for s0 in sites)),
+                   "mean_final_psnr_db": round(float(np.mean([j["final_psnr"] for j in last["jobs"]])), 3)},
+        "single_image_path": {"value": round(single, 3), "unit": "MP-iters/s", "seconds": round(single_s, 3),
+                              "max_final_psnr_diff_db": round(dpsnr, 4)},
+        "speedup_vs_single_image_path": round(value / single, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pixel_grad_batch", "avg_launch_us": round(1e3 * gms / max(1, last["grad_launches"]), 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096"])
+    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64"])
     ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -119,6 +183,8 @@ def main() -> int:
 
     if args.workload == "render4096":
         return kernel_workload(args)
+    if args.workload == "batch64":
+        return batch_workload(args)
 
     import torch
 

@@ -115,6 +115,17 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1,
                                  int height, int cand_width, int cand_height,
                                  sadh_stream_t stream);
 
+/* The tiled gradient of sadh_compute_gradients_tiled accumulated into AoS
+ * lines: grad_lines is [site_count][16] floats, values 0-9 of site i in Site
+ * field order (pos.x, pos.y, log_tau, radius, r, g, b, dir.x, dir.y,
+ * log_aniso), 10-15 unused.  No counterpart in the reference, which adds into
+ * ten arrays: here one site's ten atomic adds from one tile are one 64-byte
+ * memory request.  The fit uses this layout internally. */
+int sadh_compute_gradients_tiled_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target,
+                                     const float* mask, float* grad_lines, const SadSite* sites, float inv_scale_sq,
+                                     uint32_t site_count, float* removal_delta, uint32_t compute_removal, int width,
+                                     int height, int cand_width, int cand_height, sadh_stream_t stream);
+
 /* launchTauDiffuse, tau_diffusion.cu:388-401 */
 int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite* sites,
                      const float* grad_raw, const float* grad_in, float* grad_out,

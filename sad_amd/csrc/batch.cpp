@@ -37,10 +37,10 @@ struct JobRun {
     float mask_sum = 0.0f, iss = 0.0f, delta_norm = 0.0f;
     DevArena mem;
     float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr, *d_mse = nullptr;
-    float *tau_raw = nullptr, *tau_tmp = nullptr, *removal = nullptr;
+    float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr, *removal = nullptr;
     SadSite* d_sites = nullptr;
     SadAdamState* d_adam = nullptr;
-    float* g[10] = {};
+    float* gacc = nullptr;
     float* st[8] = {};
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count = nullptr;
@@ -102,9 +102,10 @@ int setup_job(const SadhFitOptions& base, JobRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.d_count, 1));
     SAD_TRY(J.mem.alloc(&J.d_sites, cap));
     SAD_TRY(J.mem.alloc(&J.d_adam, cap));
-    for (int j = 0; j < 10; ++j) SAD_TRY(J.mem.alloc(&J.g[j], cap));
+    SAD_TRY(J.mem.alloc(&J.gacc, (size_t)cap * 16));
     SAD_TRY(J.mem.alloc(&J.tau_raw, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp, cap));
+    SAD_TRY(J.mem.alloc(&J.tau_tmp2, cap));
     SAD_TRY(J.mem.alloc(&J.removal, cap));
     if (o.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(J.mem.alloc(&J.st[j], cap));
     if (J.pairs_count > 0) {
@@ -131,7 +132,7 @@ int setup_job(const SadhFitOptions& base, JobRun& J, hipStream_t S, bool use_hil
     HIP_TRY(hipMemcpyAsync(J.d_mask, mask.data(), sizeof(float) * J.np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(J.d_sites, hs.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(J.d_adam, 0, sizeof(SadAdamState) * cap, S));
-    for (int j = 0; j < 10; ++j) HIP_TRY(hipMemsetAsync(J.g[j], 0, sizeof(float) * cap, S));
+    HIP_TRY(hipMemsetAsync(J.gacc, 0, sizeof(float) * 16 * cap, S));
     HIP_TRY(hipMemsetAsync(J.removal, 0, sizeof(float) * cap, S));
     J.actual = io.n_sites;
     J.n = (uint32_t)J.actual;
@@ -164,9 +165,10 @@ void describe(const JobRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.sites = J.d_sites;
     d.adam = J.d_adam;
     d.search = J.search;
-    for (int k = 0; k < 10; ++k) d.g[k] = J.g[k];
+    d.gacc = J.gacc;
     d.tau_raw = J.tau_raw;
     d.tau_tmp = J.tau_tmp;
+    d.tau_tmp2 = J.tau_tmp2;
     d.removal = J.removal;
     d.h_order = J.h_order;
     d.h_pos = J.h_pos;

@@ -33,9 +33,10 @@ struct BatchJob {
     SadSite* sites;
     SadAdamState* adam;
     SadSearchSite* search;
-    float* g[10];
+    float* gacc;               /* gradients: AoS [cap][16] lines, values 0-9 in Site field order */
     float* tau_raw;
     float* tau_tmp;
+    float* tau_tmp2;
     float* removal;            /* prune iterations: removal deltas */
     const uint32_t* h_order;   /* null: no Hilbert probes */
     const uint32_t* h_pos;

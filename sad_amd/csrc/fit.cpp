@@ -161,10 +161,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     }
 
     DevArena mem;
-    float *d_target, *d_mask, *d_render, *d_mse, *tau_raw, *tau_tmp, *removal;
+    float *d_target, *d_mask, *d_render, *d_mse, *tau_raw, *tau_tmp, *tau_tmp2, *removal;
     SadSite* d_sites;
     SadAdamState* d_adam;
-    float* g[10];
+    float* gacc;   /* gradients as AoS [cap][16] lines: one 64-byte atomic target per site */
     float* st[8] = {};
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count;
@@ -177,9 +177,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&d_count, 1));
     SAD_TRY(mem.alloc(&d_sites, cap));
     SAD_TRY(mem.alloc(&d_adam, cap));
-    for (int j = 0; j < 10; ++j) SAD_TRY(mem.alloc(&g[j], cap));
+    SAD_TRY(mem.alloc(&gacc, (size_t)cap * 16));
     SAD_TRY(mem.alloc(&tau_raw, cap));
     SAD_TRY(mem.alloc(&tau_tmp, cap));
+    SAD_TRY(mem.alloc(&tau_tmp2, cap));
     SAD_TRY(mem.alloc(&removal, cap));
     if (o.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(mem.alloc(&st[j], cap));
     if (pairs_count > 0) {
@@ -217,7 +218,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     HIP_TRY(hipMemcpyAsync(d_mask, mask.data(), sizeof(float) * np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(d_sites, host_sites.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(d_adam, 0, sizeof(SadAdamState) * cap, S));
-    for (int j = 0; j < 10; ++j) HIP_TRY(hipMemsetAsync(g[j], 0, sizeof(float) * cap, S));
+    HIP_TRY(hipMemsetAsync(gacc, 0, sizeof(float) * 16 * cap, S));
     HIP_TRY(hipMemsetAsync(removal, 0, sizeof(float) * cap, S));
 
     int actual = n0;
@@ -329,8 +330,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
             gev.push_back(b);
             HIP_TRY(hipEventRecord(a, S));
         }
-        SAD_TRY(launch_gradients(cA0, cA1, d_target, d_mask, g, d_sites, iss, n, removal, prune, width, height, cw, ch,
-                                 S));
+        SAD_TRY(launch_gradients_aos(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width, height,
+                                     cw, ch, S));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             gbytes.push_back(48.0 * np + 120.0 * n + (prune ? 8.0 * n : 0.0));
@@ -342,10 +343,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         const float blend = (float)iter / (float)std::max(1, o.iterations);
         const float lambda = 0.05f * (0.1f + 0.9f * blend);
-        /* 4 tau passes (g -> tmp -> g -> tmp -> g, raw saved by the first), the last fused with Adam + clear */
-        SAD_TRY(launch_tau_adam(cA0, cA1, tau_raw, tau_tmp, lambda, cw, ch, ds, d_sites, d_adam, g, lr_pos, lr_tau,
-                                lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps, (uint32_t)(iter + 1), n, width,
-                                height, S));
+        /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
+        SAD_TRY(launch_tau_adam_aos(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites, d_adam,
+                                    lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
+                                    (uint32_t)(iter + 1), n, width, height, S));
 
         if (prune) {
             n_prune++;

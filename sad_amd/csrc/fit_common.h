@@ -94,6 +94,13 @@ struct StreamPair {
     }
 };
 
+/* AoS [N][16] gradient lines (the fit's accumulation layout) */
+int launch_gradients_aos(const uint32_t*, const uint32_t*, const float*, const float*, float*, const SadSite*, float,
+                         uint32_t, float*, bool, int, int, int, int, hipStream_t);
+int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
+                        SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
+                        uint32_t, int, int, hipStream_t);
+
 /* batched fit (one launch over every job) */
 int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);
 int launch_vpt_batch(const BatchJob*, uint32_t, uint32_t, uint32_t, const BatchStep&, hipStream_t);

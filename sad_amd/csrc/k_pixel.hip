@@ -86,17 +86,18 @@ struct TileSites {
 template <int H, bool REMOVAL, int NV>
 struct TileLds : TileSites<H> {
     static constexpr uint32_t MAX_CHUNKS = 2048u / CHUNK + (uint32_t)H;
-    uint32_t start[H];         /* first entry of each slot in `list` */
+    uint32_t start[H];         /* first entry of each slot in `list` (first: overflow site ids) */
     uint16_t cstart[H];        /* first chunk of each slot */
     uint32_t n_chunks;
+    uint32_t n_ovf;            /* pairs whose id found no hash slot */
     uint8_t slot[256][8];      /* per pixel: slot of each candidate rank (SLOT8_NONE if unused) */
     float w[256][8];           /* per pixel: softmax weights */
     float4 px[256];            /* per pixel: pred.rgb, err (STATS) */
     float4 pg[256];            /* per pixel: dL/dpred.rgb */
     float4 pt[REMOVAL ? 256 : 1];  /* per pixel: target.rgb (removal deltas) */
-    uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST) */
+    uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST); first: overflow pairs */
     uint8_t chunk_slot[MAX_CHUNKS];
-    float part[MAX_CHUNKS][NV];    /* per-chunk partial sums */
+    float part[MAX_CHUNKS][NV];    /* per-chunk partial sums; first: overflow pair values */
 };
 
 struct SiteTerms {
@@ -163,8 +164,9 @@ struct PixelArgs {
     const float* target;       /* GRAD, STATS */
     const float* mask;         /* GRAD, STATS */
     const float* grad_out;     /* BWD */
-    float* out;                /* RENDER: [H][W][3]; BWD: [N][10] */
-    float* g[10];              /* GRAD: 10 SoA arrays; STATS: 8 arrays */
+    float* out;                /* RENDER: [H][W][3] */
+    float* g[10];              /* value j of site i accumulates at g[j] + i * gstride */
+    uint32_t gstride;          /* 1: SoA arrays; 10: AoS [N][10] (BWD); 16: AoS [N][16] lines */
     float* removal;            /* GRAD with removal */
     float iss, inv_scale;
     uint32_t n;
@@ -193,6 +195,9 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
         L.key[s] = SAD_EMPTY;
         L.count[s] = 0u;
+    }
+    if constexpr (MODE != MODE_RENDER) {
+        if (tid == 0u) L.n_ovf = 0u;
     }
     __syncthreads();
 
@@ -323,43 +328,81 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         sl[k] = (use && s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
         rank[k] = (sl[k] != SLOT8_NONE) ? atomicAdd(&L.count[s], 1u) : 0u;
         L.slot[tid][k] = sl[k];
-        if (use && s == SLOT_OVERFLOW) {
-            /* an id without a hash slot: finish its contribution here, in global memory */
-            SiteTerms t = site_terms(A.sites, ids[k], A.inv_scale);
-            bool first = true;
-            float wt = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (ids[j] == ids[k] && valid[j]) {
-                    wt += w[j];
-                    if (j < k) first = false;
-                }
-            }
-            const uint32_t id = ids[k];
-            if (MODE == MODE_STATS) {
-                if (first && wt > 0.0f) {
-                    float we = wt * fminf(err, 1.0f);
-                    float v[8] = {wt, wt * err, we, we * ux, we * uy, we * ux * ux, we * ux * uy, we * uy * uy};
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) unsafeAtomicAdd(A.g[j] + id, v[j]);
-                }
-            } else {
-                float c[10];
-                candidate_grad(t, ux, uy, w[k], gr, gg, gb, pr, pgr, pb, A.iss, A.inv_scale, c);
-                if (MODE == MODE_GRAD) {
-#pragma unroll
-                    for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.g[j] + id, c[j]);
-                    if (REMOVAL && first && wt > 0.0f)
-                        unsafeAtomicAdd(A.removal + id,
-                                        sad_removal_delta(pr, pgr, pb, tr, tg, tb, t.p2.x, t.p2.y, t.p2.z, wt));
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 10; ++j) unsafeAtomicAdd(A.out + 10u * id + (uint32_t)j, c[j]);
-                }
-            }
-        }
+        if (use && s == SLOT_OVERFLOW) L.list[atomicAdd(&L.n_ovf, 1u)] = (uint16_t)((tid << 3) | (uint32_t)k);
     }
     __syncthreads();
+
+    /* stage 3c: pairs whose id found no hash slot (many only while the candidate
+     * field is still random).  Per round of R pairs, one lane differentiates a
+     * pair into LDS, then the values are issued with consecutive lanes on
+     * consecutive values of one site, so a site's values in an AoS layout are one
+     * memory request (atomics to one 64-byte line in one instruction coalesce). */
+    {
+        constexpr uint32_t R = (uint32_t)H < 128u ? (uint32_t)H : 128u;
+        const uint32_t n_ovf = L.n_ovf;
+        float* stage = &L.part[0][0];
+        for (uint32_t base = 0; base < n_ovf; base += R) {
+            const uint32_t cnt = min(R, n_ovf - base);
+            if (tid < cnt) {
+                const uint32_t ent = L.list[base + tid];
+                const uint32_t p = ent >> 3, k = ent & 7u;
+                const int ex = tx0 + (int)(p & 15u), ey = ty0 + (int)(p >> 4);
+                const int cell = sad_cand_index(ex, ey, A.width, A.height, A.cw, A.ch);
+                const uint4 ca = reinterpret_cast<const uint4*>(A.cand0)[cell];
+                const uint4 cb = reinterpret_cast<const uint4*>(A.cand1)[cell];
+                const uint32_t pid[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+                const uint32_t id = pid[k];
+                /* a repeated id shares validity with this one; invalid weights are 0 */
+                bool first = true;
+                float wt = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (pid[j] == id) {
+                        wt += L.w[p][j];
+                        if ((uint32_t)j < k) first = false;
+                    }
+                }
+                const float4 pp = L.px[p];
+                const float4 gq = L.pg[p];
+                const float fx = (float)ex, fy = (float)ey;
+                float v[NV];
+#pragma unroll
+                for (int j = 0; j < NV; ++j) v[j] = 0.0f;
+                if constexpr (MODE == MODE_STATS) {
+                    if (first && wt > 0.0f) {
+                        const float er = pp.w;
+                        const float we = wt * fminf(er, 1.0f);
+                        v[0] = wt; v[1] = wt * er; v[2] = we; v[3] = we * fx; v[4] = we * fy;
+                        v[5] = we * fx * fx; v[6] = we * fx * fy; v[7] = we * fy * fy;
+                    }
+                } else {
+                    const SiteTerms t = site_terms(A.sites, id, A.inv_scale);
+                    float c[10];
+                    candidate_grad(t, fx, fy, L.w[p][k], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss, A.inv_scale, c);
+#pragma unroll
+                    for (int j = 0; j < 10; ++j) v[j] = c[j];
+                    if (MODE == MODE_GRAD && REMOVAL && first && wt > 0.0f) {
+                        const float4 tq = L.pt[p];
+                        v[NV - 1] = sad_removal_delta(pp.x, pp.y, pp.z, tq.x, tq.y, tq.z, t.p2.x, t.p2.y, t.p2.z, wt);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NV; ++j) stage[tid * (uint32_t)NV + (uint32_t)j] = v[j];
+                L.start[tid] = id;
+            }
+            __syncthreads();
+            for (uint32_t q = tid; q < cnt * (uint32_t)NV; q += 256u) {
+                const uint32_t e = q / (uint32_t)NV, j = q - e * (uint32_t)NV;
+                const float v = stage[q];
+                if (v != 0.0f) {
+                    const uint32_t id = L.start[e];
+                    if (MODE == MODE_GRAD && REMOVAL && j == 10u) unsafeAtomicAdd(A.removal + id, v);
+                    else unsafeAtomicAdd(A.g[j] + id * A.gstride, v);
+                }
+            }
+            __syncthreads();
+        }
+    }
 
     /* stage 4 (wave 0): entry offsets and chunk offsets = exclusive scans of the
      * per-slot entry and chunk counts; chunk -> slot table */
@@ -470,12 +513,10 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         float v = 0.0f;
         for (uint32_t q = q0; q < q1; ++q) v += L.part[q][j];
         const uint32_t id = L.key[s];
-        if (MODE == MODE_BWD) {
-            unsafeAtomicAdd(A.out + 10u * id + j, v);
-        } else if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
+        if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
             if (v != 0.0f) unsafeAtomicAdd(A.removal + id, v);
         } else {
-            unsafeAtomicAdd(A.g[j] + id, v);
+            unsafeAtomicAdd(A.g[j] + id * A.gstride, v);
         }
     }
     }  /* backward modes */
@@ -504,7 +545,8 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.target = J.target;
     A.mask = J.mask;
 #pragma unroll
-    for (int k = 0; k < 10; ++k) A.g[k] = J.g[k];
+    for (int k = 0; k < 10; ++k) A.g[k] = J.gacc + k;
+    A.gstride = 16u;
     A.removal = J.removal;
     A.iss = J.iss;
     A.inv_scale = J.inv_scale;
@@ -551,10 +593,26 @@ int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* 
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
+    A.gstride = 1u;
     A.removal = removal; A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     if (compute_removal) return launch_pixel<MODE_GRAD, true>(A, stream, "compute_gradients_tiled");
     return launch_pixel<MODE_GRAD, false>(A, stream, "compute_gradients_tiled");
+}
+
+/* gradients into AoS [N][16] lines (values 0-9 in Site field order) */
+int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                         float* gacc, const SadSite* sites, float iss, uint32_t n, float* removal, bool compute_removal,
+                         int width, int height, int cw, int ch, hipStream_t stream)
+{
+    PixelArgs A = {};
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
+    for (int j = 0; j < 10; ++j) A.g[j] = gacc + j;
+    A.gstride = 16u;
+    A.removal = removal; A.iss = iss; A.n = n;
+    A.width = width; A.height = height; A.cw = cw; A.ch = ch;
+    if (compute_removal) return launch_pixel<MODE_GRAD, true>(A, stream, "compute_gradients_tiled(aos)");
+    return launch_pixel<MODE_GRAD, false>(A, stream, "compute_gradients_tiled(aos)");
 }
 
 int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
@@ -577,6 +635,7 @@ int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float*
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
     for (int j = 0; j < 8; ++j) A.g[j] = st[j];
+    A.gstride = 1u;
     A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     return launch_pixel<MODE_STATS, false>(A, stream, "compute_site_stats");
 }
@@ -606,7 +665,9 @@ int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const flo
     SADH_REQUIRE(cand0 && cand1 && grad_output && grad_sites && sites, "render_backward: null pointer");
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_backward: bad sizes");
     PixelArgs A = {};
-    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.grad_out = grad_output; A.out = grad_sites;
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.grad_out = grad_output;
+    for (int j = 0; j < 10; ++j) A.g[j] = grad_sites + j;
+    A.gstride = 10u;
     A.iss = inv_scale_sq; A.n = site_count; A.width = width; A.height = height;
     A.cw = cand_width; A.ch = cand_height;
     return launch_pixel<MODE_BWD, false>(A, to_hip(stream), "render_backward");
@@ -632,6 +693,19 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, c
                           grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
     return launch_gradients(cand0, cand1, target, mask, g, sites, inv_scale_sq, site_count, removal_delta,
                             compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream));
+}
+
+int sadh_compute_gradients_tiled_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target,
+                                     const float* mask, float* grad_lines, const SadSite* sites, float inv_scale_sq,
+                                     uint32_t site_count, float* removal_delta, uint32_t compute_removal, int width,
+                                     int height, int cand_width, int cand_height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(cand0 && cand1 && target && mask && sites && grad_lines, "compute_gradients_tiled_aos: null pointer");
+    SADH_REQUIRE(!compute_removal || removal_delta, "compute_gradients_tiled_aos: removal_delta missing");
+    SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0,
+                 "compute_gradients_tiled_aos: bad sizes");
+    return launch_gradients_aos(cand0, cand1, target, mask, grad_lines, sites, inv_scale_sq, site_count, removal_delta,
+                                compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream));
 }
 
 int sadh_compute_site_stats_simple(const uint32_t* cand0, const uint32_t* cand1, const float* target,

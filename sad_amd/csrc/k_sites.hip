@@ -19,7 +19,8 @@ namespace sadh {
 struct AdamArgs {
     SadSite* sites;
     SadAdamState* st;
-    float* g[10];
+    float* g[10];              /* gradient j of site i at g[j][i * gstride] */
+    uint32_t gstride;          /* 1: SoA arrays; 16: AoS [N][16] lines */
     float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
     float beta1, beta2, eps, b1c, b2c, norm, maxx, maxy;
     uint32_t n;
@@ -95,10 +96,10 @@ __global__ __launch_bounds__(256) void k_adam(AdamArgs A)
     if (i >= A.n) return;
     float g[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) g[j] = A.g[j][i] * A.norm;
+    for (int j = 0; j < 10; ++j) g[j] = A.g[j][i * A.gstride] * A.norm;
     if (CLEAR) {
 #pragma unroll
-        for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
+        for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
     }
     adam_site(A, i, g);
 }
@@ -126,13 +127,14 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
                                          const SadSite* __restrict__ sites, const float* __restrict__ graw,
                                          const float* __restrict__ gin, float* __restrict__ gout,
                                          float* __restrict__ raw_copy, uint32_t n, float lambda, int width,
-                                         int height, float inv_ds)
+                                         int height, float inv_ds, uint32_t in_stride = 1u)
 {
-    float raw = graw[i];
+    /* graw and gin hold site i at [i * in_stride]; gout and raw_copy are contiguous */
+    float raw = graw[i * in_stride];
     if (raw_copy) raw_copy[i] = raw;
     const float2* sp = reinterpret_cast<const float2*>(sites + i);
     float2 pos = sp[0];
-    if (lambda <= 0.0f || pos.x < 0.0f) { gout[i] = gin[i]; return; }
+    if (lambda <= 0.0f || pos.x < 0.0f) { gout[i] = gin[i * in_stride]; return; }
     int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
     int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
     uint4 a = c0[py * width + px], b = c1[py * width + px];
@@ -146,12 +148,23 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
 #pragma unroll
         for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
         if (ok && sites[nb].pos_x >= 0.0f) {
-            sum += gin[nb];
+            sum += gin[nb * in_stride];
             cnt += 1u;
         }
     }
     float diag = 1.0f + lambda * (float)cnt;
     gout[i] = (raw + lambda * sum) / diag;
+}
+
+/* first pass reading the log_tau column of AoS [N][16] gradient lines */
+__global__ __launch_bounds__(256) void k_tau_pass_strided(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                          const SadSite* __restrict__ sites,
+                                                          const float* __restrict__ g2, float* __restrict__ gout,
+                                                          float* __restrict__ raw_copy, uint32_t n, float lambda,
+                                                          int width, int height, float inv_ds)
+{
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) tau_site(i, c0, c1, sites, g2, g2, gout, raw_copy, n, lambda, width, height, inv_ds, 16u);
 }
 
 __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
@@ -206,9 +219,9 @@ __device__ __forceinline__ void tau_last_adam_site(uint32_t i, const uint4* __re
     }
     float g[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i]) * A.norm;
+    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
 #pragma unroll
-    for (int j = 0; j < 10; ++j) A.g[j][i] = 0.0f;
+    for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
     adam_site(A, i, g);
 }
 
@@ -220,9 +233,10 @@ __global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__
     if (i < A.n) tau_last_adam_site(i, c0, c1, graw, gin, lambda, width, height, inv_ds, A);
 }
 
-/* Batched fit: tau pass `pass` (0-2: g2 -> tmp -> g2 -> tmp, raw saved by pass
- * 0) or, for pass 3, the last pass fused with Adam + clear, over the
- * concatenated sites of every job; the candidate field is the current one. */
+/* Batched fit: tau pass `pass` (0-2: log_tau column of the gradient lines ->
+ * tmp -> tmp2 -> tmp, raw saved by pass 0) or, for pass 3, the last pass fused
+ * with Adam + clear, over the concatenated sites of every job; the candidate
+ * field is each job's current one. */
 __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t pass,
                                                    BatchStep st)
 {
@@ -234,18 +248,19 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
     const uint4* c0 = reinterpret_cast<const uint4*>(J.cur ? J.fb0 : J.fa0);
     const uint4* c1 = reinterpret_cast<const uint4*>(J.cur ? J.fb1 : J.fa1);
     const float inv_ds = 1.0f / fmaxf((float)J.ds, 1.0f);
-    float* g2 = J.g[2];
     if (pass == 0u) {
-        tau_site(i, c0, c1, J.sites, g2, g2, J.tau_tmp, J.tau_raw, J.n, st.lambda, J.cw, J.ch, inv_ds);
+        tau_site(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, st.lambda, J.cw, J.ch, inv_ds,
+                 16u);
     } else if (pass == 1u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, g2, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, J.tau_tmp2, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
     } else if (pass == 2u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, g2, J.tau_tmp, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp2, J.tau_tmp, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
     } else {
         AdamArgs A;
         A.sites = J.sites; A.st = J.adam;
 #pragma unroll
-        for (int k = 0; k < 10; ++k) A.g[k] = J.g[k];
+        for (int k = 0; k < 10; ++k) A.g[k] = J.gacc + k;
+        A.gstride = 16u;
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
         A.b1c = st.b1c; A.b2c = st.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
@@ -480,6 +495,7 @@ static void fill_adam(AdamArgs& A, SadSite* sites, SadAdamState* st, float* cons
 {
     A.sites = sites; A.st = st;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
+    A.gstride = 1u;
     A.lr_pos = lr_pos; A.lr_tau = lr_tau; A.lr_radius = lr_radius; A.lr_color = lr_color;
     A.lr_dir = lr_dir; A.lr_aniso = lr_aniso; A.beta1 = beta1; A.beta2 = beta2; A.eps = eps;
     A.b1c = 1.0f / (1.0f - powf(beta1, (float)t));
@@ -554,6 +570,38 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
         hipLaunchKernelGGL(k_update_jobs, dim3(1), dim3(64), 0, stream, jobs, u);
     }
     return launch_status("batch_update_jobs");
+}
+
+/* The fit's tail on AoS [N][16] gradient lines: pass 1 reads the strided
+ * log_tau column (raw saved), passes 2-3 ping-pong tmp -> tmp2 -> tmp, pass 4
+ * feeds Adam, which reads and clears each site's line.  Same values as the
+ * SoA chain (the buffers differ, not the arithmetic). */
+int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, float* raw, float* tmp, float* tmp2,
+                        float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
+                        float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
+                        float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
+                        hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    const float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
+    const dim3 grid(div_up(n, 256u));
+    const uint4* u0 = reinterpret_cast<const uint4*>(c0);
+    const uint4* u1 = reinterpret_cast<const uint4*>(c1);
+    hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
+                       cand_width, cand_height, inv_ds);
+    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp, tmp2, (float*)nullptr, n,
+                       lambda, cand_width, cand_height, inv_ds);
+    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp2, tmp, (float*)nullptr, n,
+                       lambda, cand_width, cand_height, inv_ds);
+    float* g[10];
+    for (int j = 0; j < 10; ++j) g[j] = gacc + j;
+    AdamArgs A;
+    fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
+              height);
+    A.gstride = 16u;
+    hipLaunchKernelGGL(k_tau_last_adam, grid, dim3(256), 0, stream, u0, u1, raw, tmp, lambda, cand_width,
+                       cand_height, inv_ds, A);
+    return launch_status("tau_diffuse_adam(aos)");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)

@@ -126,3 +126,25 @@ def fit_workload(name: str):
         over = dict(iterations=w["iterations"], n_sites=w["n_sites"], target_bpp=1.0, densify_enabled=1,
                     densify_freq=100, prune_freq=100, max_sites=262144)
     return tgt, sites, over
+
+
+def batch_workload(n_images: int = 64, seed: int = 4, min_side: int = 128, max_side: int = 2048,
+                   bits_per_site: float = 128.0, bpp: float = 1.0):
+    """The batched-fit workload: `n_images` seeded targets whose base side s is
+    log-uniform in [min_side, max_side] and aspect ratio a log-uniform in
+    [0.5, 2] (W = s*sqrt(a), H = s/sqrt(a), rounded), each with
+    round(bpp * W * H / bits_per_site) uniform initial sites.  Returns
+    (targets, sites)."""
+    rng = np.random.default_rng(seed)
+    targets, sites = [], []
+    for i in range(n_images):
+        side = float(np.exp(rng.uniform(np.log(min_side), np.log(max_side))))
+        asp = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+        w = max(16, int(round(side * np.sqrt(asp))))
+        h = max(16, int(round(side / np.sqrt(asp))))
+        t = make_target(w, h, seed * 1000 + i, n_ellipses=40)
+        n = max(16, int(round(bpp * w * h / bits_per_site)))
+        targets.append(t)
+        sites.append(init_sites_uniform(t, n, seed * 1000 + i))
+    return targets, sites
+

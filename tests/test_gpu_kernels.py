@@ -220,6 +220,30 @@ def test_gradients_reject_ssim(gpu):
         gpu.call("sadh_compute_gradients_tiled", z, z, z, z, z, *([z] * 10), z, 1.0, 1, z, 0, 0.5, 2, 2, 2, 2)
 
 
+@pytest.mark.parametrize("removal", [False, True])
+@pytest.mark.parametrize("w,h,n,passes", [(160, 128, 900, 12), (256, 256, 2048, 0)])
+def test_gradients_aos_parity(gpu, orc, w, h, n, passes, removal):
+    """AoS [N][16] gradient lines against the oracle (passes=0: random field,
+    nearly every pair overflows the tile hash)."""
+    sites, c0, c1 = _scene(orc, w, h, n, 10, passes=passes)
+    rng = np.random.default_rng(11)
+    target = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    mask = np.ones((h, w), np.float32)
+    mask[rng.random((h, w)) < 0.1] = 0.0
+    iss = iss_of(w, h)
+    lines = gpu.zeros((n, 16))
+    rd = gpu.zeros((n,))
+    g0, g1 = _fields(gpu, c0, c1)
+    gpu.call("sadh_compute_gradients_tiled_aos", g0, g1, gpu.put(target), gpu.put(mask), lines, gpu.put(sites), iss,
+             n, rd, int(removal), w, h, w, h)
+    acc, ab, rdr, rda = orc.gradients(c0, c1, target, mask, sites, iss, w, h, compute_removal=removal)
+    got = gpu.get(lines)
+    assert np.all(got[:, 10:] == 0)
+    assert Rel(got[:, :10], acc, ab) <= 1.0
+    if removal:
+        assert Rel(gpu.get(rd), rdr, rda) <= 1.0
+
+
 def test_render_backward_parity(gpu, orc):
     w, h, n = 144, 112, 800
     sites, c0, c1 = _scene(orc, w, h, n, 12)
