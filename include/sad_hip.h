@@ -315,6 +315,41 @@ int sadh_fit(const SadhFitOptions* opts, const float* target, const float* mask,
  * one launch over all jobs (a device table of per-image descriptors). */
 int sadh_fit_batch(const SadhFitOptions* opts, SadhBatchJob* jobs, int n_jobs, SadhBatchResult* result);
 
+/* Continuous-batching server (no reference counterpart: the reference serves
+ * one fit or render per process).  Fit requests (SadhBatchJob, options shared
+ * through SadhServerOptions.fit, each with its own iteration counter) and
+ * render requests (sites -> image: clear, JFA seed + flood, VPT passes,
+ * render) wait in arrival order and are admitted while their projected device
+ * memory fits the budget and a fit slot is free.  sadh_server_step admits what
+ * fits, advances every running fit by one iteration in batched launches, lets
+ * render requests run on their own stream (overlapping the fits), and retires
+ * finished requests; sadh_server_poll returns their events.  The host buffers
+ * of a request must stay valid until its event. */
+typedef struct SadhServerOptions {
+    SadhFitOptions fit;               /* shared by every fit request (n_sites, init_mode ignored) */
+    double memory_budget;             /* bytes; <= 0: 0.8 x the device's memory */
+    int max_fit_jobs;                 /* running fits at most; <= 0: 64 */
+    int render_passes;                /* VPT passes of a render request after JFA; <= 0: 16 */
+} SadhServerOptions;
+
+typedef struct SadhServerEvent {
+    long long id;
+    int kind;                         /* 0 fit, 1 render */
+    int status;                       /* SADH_OK */
+    double submit_s, admit_s, done_s; /* seconds on the server's clock */
+} SadhServerEvent;
+
+typedef struct SadhServer* sadh_server_t;
+
+int sadh_server_create(const SadhServerOptions* opts, sadh_server_t* server);
+int sadh_server_submit_fit(sadh_server_t server, const SadhBatchJob* job, long long* id);
+int sadh_server_submit_render(sadh_server_t server, const SadSite* sites, uint32_t site_count, int width, int height,
+                              float* output, long long* id);
+int sadh_server_step(sadh_server_t server, int* outstanding);
+int sadh_server_poll(sadh_server_t server, SadhServerEvent* events, int cap, int* count);
+double sadh_server_time(sadh_server_t server);   /* seconds since creation (the events' clock) */
+int sadh_server_destroy(sadh_server_t server);
+
 /* Render mode of renderVoronoiFromSites, sad_cuda.cu:1678-1858: JFA (jfa_rounds
  * rounds of clear + seed + log2 floods) and `cand_passes` VPT passes, then the
  * softmax render.  sites and output are host arrays; timings in ms (nullable). */

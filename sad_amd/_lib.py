@@ -65,6 +65,17 @@ class BatchResult(C.Structure):
     ]
 
 
+class ServerOptions(C.Structure):
+    """SadhServerOptions (include/sad_hip.h)."""
+    _fields_ = [("fit", FitOptions), ("memory_budget", C.c_double), ("max_fit_jobs", I32), ("render_passes", I32)]
+
+
+class ServerEvent(C.Structure):
+    """SadhServerEvent (include/sad_hip.h)."""
+    _fields_ = [("id", C.c_longlong), ("kind", I32), ("status", I32), ("submit_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
 # name -> argument types (all return int)
 SIGNATURES = {
     "sadh_render_voronoi": [P, P, P, P, F32, U32, I32, I32, I32, I32, P],
@@ -99,6 +110,12 @@ SIGNATURES = {
                       C.POINTER(I32)],
     "sadh_fit": [C.POINTER(FitOptions), P, P, I32, I32, P, I32, P, I32, C.POINTER(FitResult)],
     "sadh_fit_batch": [C.POINTER(FitOptions), C.POINTER(BatchJob), I32, C.POINTER(BatchResult)],
+    "sadh_server_create": [C.POINTER(ServerOptions), C.POINTER(P)],
+    "sadh_server_submit_fit": [P, C.POINTER(BatchJob), C.POINTER(C.c_longlong)],
+    "sadh_server_submit_render": [P, P, U32, I32, I32, P, C.POINTER(C.c_longlong)],
+    "sadh_server_step": [P, C.POINTER(I32)],
+    "sadh_server_poll": [P, C.POINTER(ServerEvent), I32, C.POINTER(I32)],
+    "sadh_server_destroy": [P],
     "sadh_render_sites": [P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
                           C.POINTER(F32)],
 }
@@ -125,6 +142,8 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    lib.sadh_server_time.restype = C.c_double
+    lib.sadh_server_time.argtypes = [P]
     lib.sadh_last_error.restype = C.c_char_p
     lib.sadh_last_error.argtypes = []
     lib.sadh_version.restype = C.c_char_p

@@ -1,25 +1,36 @@
 /*
- * batch.cpp - batched fit: many images, one shared option set, advanced in
- * lock step (the reference fits one image per process, sad_cuda.cu:1860-2533).
+ * batch.cpp - the batching engine: many fits on one device, each at its own
+ * iteration, advanced together one "tick" (one iteration of every active fit)
+ * at a time; and the continuous-batching server built on it.
  *
- * Every iteration issues one launch per kernel for the whole batch: the VPT
- * pass (with its search-site preparation), the tiled gradient and the four tau
- * passes (the last fused with Adam + clear) each run over the concatenation of
- * all jobs' tiles or sites, finding their job through the device descriptor
- * table (batch.h).  The rare iterations keep per-job launches: densify (site
- * statistics, score pairs, sort, split), prune, Hilbert tables and the log
- * render + PSNR.
+ * A tick issues one launch per kernel for all active fits: the VPT pass (with
+ * its search-site preparation) for the fits that update candidates this tick,
+ * the tiled gradient, and the four tau passes (the last fused with Adam +
+ * clear).  Each launch runs over the concatenation of the fits' tiles or sites
+ * and finds its fit through a device table of descriptors (batch.h); the
+ * per-fit iteration scalars (tau weight, Adam bias corrections, VPT jump index,
+ * "update now") come from a small per-tick table.  Rare per-fit work keeps its
+ * own launches: densify (site statistics, score pairs, sort, split), prune,
+ * Hilbert tables, and the log render + PSNR.
  *
- * Each job follows exactly the schedule of the single-image fit (fit.cpp),
- * including the overlap semantics: on a candidate-update iteration a job that
- * does not densify computes its gradient from the field before the update
- * (the VPT runs on a second stream), a job that densifies updates its field
- * first.  Which field is current is a per-job flag in the descriptor, flipped
- * in stream order, so a mixed batch keeps both behaviours.
+ * Every fit follows exactly the schedule of the single-image fit (fit.cpp,
+ * the reference's trainVoronoi, sad_cuda.cu:1860-2533), including the overlap
+ * semantics: on a candidate-update iteration a fit that does not densify
+ * computes its gradient from the field before the update (the VPT runs on a
+ * second stream), a fit that densifies updates its field first.  Which field
+ * is current is a per-fit flag in the descriptor, flipped in stream order.
+ *
+ * sadh_fit_batch adds all fits at once and ticks until they finish.  The
+ * server (sadh_server_*) admits queued fit and render requests in arrival
+ * order while their projected device memory fits the budget (and a fit slot
+ * is free), ticks the running fits, runs render requests on a third stream so
+ * they overlap the fits, and retires finished requests with their times.
+ * The reference has neither: it fits or renders one image per process.
  */
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 
 #include "fit_common.h"
@@ -28,8 +39,13 @@ namespace sadh {
 
 namespace {
 
-struct JobRun {
-    SadhBatchJob* io = nullptr;
+using Clock = std::chrono::steady_clock;
+
+struct FitRun {
+    SadhBatchJob io = {};
+    long long id = 0;
+    int iterations = 0, iter = 0;
+    uint32_t jump_index = 0;
     float d_pct = 0.0f, p_pct = 0.0f;
     int cap = 0, pairs_count = 0;
     int width = 0, height = 0, np = 0, ds = 1, cw = 0, ch = 0;
@@ -49,33 +65,63 @@ struct JobRun {
     uint32_t *h_order = nullptr, *h_pos = nullptr, *h_hist = nullptr;
     SadPair *h_pairs = nullptr, *h_scratch = nullptr;
     uint32_t hil_padded = 0;
-    /* host state */
     int actual = 0, active_est = 0;
     uint32_t n = 0, cur = 0, hil_count = 0;
     bool hil_ready = false;
-    float best = 0.0f, final_psnr = 0.0f;
+    float best = 0.0f, final_psnr = 0.0f, sse = 0.0f;
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
+    double submit_s = 0.0, admit_s = 0.0;
+    /* this tick */
+    bool upd = false, dens = false, prune = false, log = false;
 
     uint32_t* c0() const { return cur ? fb0 : fa0; }
     uint32_t* c1() const { return cur ? fb1 : fa1; }
 };
 
-int setup_job(const SadhFitOptions& base, JobRun& J, hipStream_t S, bool use_hil)
+/* device bytes a fit allocates (the admission projection) */
+size_t fit_bytes(const SadhFitOptions& o, int width, int height, int cap, int pairs_count, bool use_hil)
 {
-    SadhBatchJob& io = *J.io;
+    const size_t np = (size_t)width * height;
+    const int ds = std::max(1, o.cand_downscale);
+    const size_t cells = (size_t)((width + ds - 1) / ds) * ((height + ds - 1) / ds);
+    size_t b = cells * 64 + np * (12 + 4 + 12);
+    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 64 + 16 + sizeof(SadSearchSite));
+    if (o.densify_enabled) b += (size_t)cap * 32;
+    if (pairs_count > 0) b += (size_t)pairs_count * 16 + 256 * 4 * (pairs_count / 1024 + 1) + 2 * 65536 * 4;
+    if (use_hil) b += (size_t)cap * 8 + 16 * (size_t)(cap / 1024 + 1) * 1024 * 2;
+    return b;
+}
+
+size_t render_bytes(uint32_t n, int width, int height, bool use_hil)
+{
+    size_t b = (size_t)width * height * (64 + 12) + (size_t)n * (sizeof(SadSite) + sizeof(SadSearchSite));
+    if (use_hil) b += (size_t)n * 8 + 16 * (size_t)(n / 1024 + 1) * 1024 * 2;
+    return b;
+}
+
+int fit_plan_for(const SadhFitOptions& base, const SadhBatchJob& io, float* d_pct, float* p_pct, int* cap,
+                 int* pairs)
+{
     SadhFitOptions o = base;
     o.n_sites = io.n_sites;
     o.init_mode = 0;
+    return plan(o, io.width, io.height, d_pct, p_pct, cap, pairs);
+}
+
+int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil)
+{
+    SadhBatchJob& io = J.io;
+    J.iterations = base.iterations;
     J.width = io.width;
     J.height = io.height;
-    SAD_TRY(plan(o, J.width, J.height, &J.d_pct, &J.p_pct, &J.cap, &J.pairs_count));
+    SAD_TRY(fit_plan_for(base, io, &J.d_pct, &J.p_pct, &J.cap, &J.pairs_count));
     if (!io.target || !io.sites_io) return fail(SADH_ERR_INVALID, "fit_batch: null target or sites");
     if (io.sites_cap < J.cap)
         return fail(SADH_ERR_CAPACITY, "fit_batch: a job's sites_io holds " + std::to_string(io.sites_cap) +
                                            " sites, its plan needs " + std::to_string(J.cap));
     J.np = J.width * J.height;
-    J.ds = std::max(1, o.cand_downscale);
+    J.ds = std::max(1, base.cand_downscale);
     J.cw = std::max(1, (J.width + J.ds - 1) / J.ds);
     J.ch = std::max(1, (J.height + J.ds - 1) / J.ds);
     J.cells = (size_t)J.cw * J.ch;
@@ -107,7 +153,7 @@ int setup_job(const SadhFitOptions& base, JobRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.tau_tmp, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp2, cap));
     SAD_TRY(J.mem.alloc(&J.removal, cap));
-    if (o.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(J.mem.alloc(&J.st[j], cap));
+    if (base.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(J.mem.alloc(&J.st[j], cap));
     if (J.pairs_count > 0) {
         SAD_TRY(J.mem.alloc(&J.pairs, J.pairs_count));
         SAD_TRY(J.mem.alloc(&J.sort_scratch, J.pairs_count));
@@ -146,7 +192,7 @@ int setup_job(const SadhFitOptions& base, JobRun& J, hipStream_t S, bool use_hil
     return SADH_OK;
 }
 
-void describe(const JobRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_begin, uint32_t site_begin)
+void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_begin, uint32_t site_begin)
 {
     std::memset(&d, 0, sizeof(d));
     d.target = J.d_target;
@@ -184,88 +230,158 @@ void describe(const JobRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.site_begin = site_begin;
 }
 
-int flip_jobs(BatchJob* d_jobs, const std::vector<uint32_t>& which, hipStream_t S)
-{
-    if (which.empty()) return SADH_OK;
-    std::vector<uint32_t> keep(which.size(), 0xFFFFFFFFu), one(which.size(), 1u);
-    return launch_update_jobs(d_jobs, which.data(), keep.data(), one.data(), (uint32_t)which.size(), S);
-}
-
-}  // namespace
-
-static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs, SadhBatchResult* bres)
-{
-    if (n_jobs <= 0 || !ios) return fail(SADH_ERR_INVALID, "fit_batch: no jobs");
-    if (o.init_mode != 0) return fail(SADH_ERR_UNSUPPORTED, "fit_batch: only caller-provided initial sites");
-    const bool use_hil = o.cand_hilbert_probes > 0 && o.cand_hilbert_window > 0;
+/* The engine: running fits, their device table, streams and per-tick tables. */
+struct Engine {
+    SadhFitOptions o = {};
+    bool use_hil = false;
     StreamPair sp;
-    HIP_TRY(hipStreamCreateWithFlags(&sp.main, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&sp.cand, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&sp.cand_done, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&sp.sites_ready, hipEventDisableTiming));
-    hipStream_t S = sp.main;
-
-    std::vector<std::unique_ptr<JobRun>> jobs;
-    for (int j = 0; j < n_jobs; ++j) {
-        jobs.emplace_back(new JobRun());
-        jobs.back()->io = &ios[j];
-        int rc = setup_job(o, *jobs.back(), S, use_hil);
-        if (rc) return rc;
-    }
-    /* descriptor table */
-    std::vector<BatchJob> desc(n_jobs);
+    hipEvent_t steps_ready = nullptr;
+    std::vector<std::unique_ptr<FitRun>> fits;
+    bool table_dirty = true;
+    DevArena tmem;                      /* device tables (a new one when the count grows) */
+    BatchJob* d_jobs = nullptr;
+    int d_jobs_cap = 0;
     uint32_t total_tiles = 0, total_ctiles = 0, total_sites = 0;
-    for (int j = 0; j < n_jobs; ++j) {
-        describe(*jobs[j], desc[j], total_tiles, total_ctiles, total_sites);
-        total_tiles += desc[j].ntiles;
-        total_ctiles += desc[j].cntiles;
-        total_sites += desc[j].cap;
-    }
-    DevArena tmem;
-    BatchJob* d_jobs;
-    SAD_TRY(tmem.alloc(&d_jobs, (size_t)n_jobs));
-    HIP_TRY(hipMemcpy(d_jobs, desc.data(), sizeof(BatchJob) * n_jobs, hipMemcpyHostToDevice));
-    HIP_TRY(hipEventRecord(sp.sites_ready, S));
-
-    const float lr_pos = o.lr_pos * o.lr_scale, lr_tau = o.lr_tau * o.lr_scale, lr_rad = o.lr_radius * o.lr_scale;
-    const float lr_col = o.lr_color * o.lr_scale, lr_dir = o.lr_dir * o.lr_scale, lr_an = o.lr_aniso * o.lr_scale;
-    int eff_ps = o.prune_start;
-    if (o.densify_enabled && !o.prune_during_densify && o.prune_start < o.densify_end) eff_ps = o.densify_end;
-    const int eff_pe = o.prune_end > 0 ? o.prune_end : o.iterations - 1;
-    uint32_t jump_index = 0;
-
+    static constexpr int RING = 4;      /* per-tick step tables: pinned host + device slots */
+    JobStep* h_steps = nullptr;
+    JobStep* d_steps = nullptr;
+    int steps_cap = 0;
+    hipEvent_t slot_done[RING] = {};
+    uint64_t tick_no = 0;
     std::vector<hipEvent_t> gev;
     std::vector<double> gbytes;
-    struct EventGuard {
-        std::vector<hipEvent_t>& v;
-        ~EventGuard() { for (auto e : v) (void)hipEventDestroy(e); }
-    } gev_guard{gev};
-    std::vector<uint32_t> upd_idx, upd_n, upd_flip, late_flip;
-    std::vector<char> dens(n_jobs), prune(n_jobs);
+    Clock::time_point t_origin = Clock::now();
 
-    auto t0 = std::chrono::steady_clock::now();
-    for (int iter = 0; iter < o.iterations; ++iter) {
-        const bool upd = o.cand_update_freq > 0 && iter % o.cand_update_freq == 0;
-        const int passes = upd ? std::max(1, o.cand_update_passes) : 0;
-        const bool dens_iter = o.densify_enabled && iter >= o.densify_start && iter <= o.densify_end &&
-                               iter % std::max(1, o.densify_freq) == 0;
-        const bool prune_iter = iter >= eff_ps && iter < eff_pe && iter % std::max(1, o.prune_freq) == 0;
-        bool any_dens = false, any_prune = false;
-        for (int j = 0; j < n_jobs; ++j) {
-            JobRun& J = *jobs[j];
-            const int desired = std::max(0, (int)((float)J.active_est * J.d_pct));
-            dens[j] = dens_iter && J.actual < J.cap && desired > 0;
-            prune[j] = prune_iter && J.p_pct > 0.0f && J.pairs_count > 0;
-            any_dens = any_dens || dens[j];
-            any_prune = any_prune || prune[j];
+    ~Engine()
+    {
+        for (auto e : gev) (void)hipEventDestroy(e);
+        for (auto& e : slot_done) if (e) (void)hipEventDestroy(e);
+        if (steps_ready) (void)hipEventDestroy(steps_ready);
+        if (h_steps) (void)hipHostFree(h_steps);
+        if (d_steps) (void)hipFree(d_steps);
+    }
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t_origin).count(); }
+
+    int init(const SadhFitOptions& opts)
+    {
+        o = opts;
+        if (o.init_mode != 0) return fail(SADH_ERR_UNSUPPORTED, "fit_batch: only caller-provided initial sites");
+        use_hil = o.cand_hilbert_probes > 0 && o.cand_hilbert_window > 0;
+        HIP_TRY(hipStreamCreateWithFlags(&sp.main, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&sp.cand, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&sp.cand_done, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&sp.sites_ready, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&steps_ready, hipEventDisableTiming));
+        for (auto& e : slot_done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return SADH_OK;
+    }
+
+    int add_fit(const SadhBatchJob& io, long long id, double submit_s)
+    {
+        std::unique_ptr<FitRun> f(new FitRun());
+        f->io = io;
+        f->id = id;
+        f->submit_s = submit_s;
+        /* the device table is read by in-flight kernels: finish them before it changes */
+        HIP_TRY(hipStreamSynchronize(sp.main));
+        HIP_TRY(hipStreamSynchronize(sp.cand));
+        SAD_TRY(setup_fit(o, *f, sp.main, use_hil));
+        f->admit_s = now();
+        fits.push_back(std::move(f));
+        table_dirty = true;
+        return SADH_OK;
+    }
+
+    int rebuild_table()
+    {
+        HIP_TRY(hipStreamSynchronize(sp.main));
+        HIP_TRY(hipStreamSynchronize(sp.cand));
+        const int nj = (int)fits.size();
+        std::vector<BatchJob> desc(nj);
+        total_tiles = total_ctiles = total_sites = 0;
+        for (int j = 0; j < nj; ++j) {
+            describe(*fits[j], desc[j], total_tiles, total_ctiles, total_sites);
+            total_tiles += desc[j].ntiles;
+            total_ctiles += desc[j].cntiles;
+            total_sites += desc[j].cap;
         }
+        if (nj > d_jobs_cap) {
+            d_jobs_cap = std::max(nj, 2 * d_jobs_cap);
+            SAD_TRY(tmem.alloc(&d_jobs, (size_t)d_jobs_cap));
+        }
+        if (nj > steps_cap) {
+            if (h_steps) (void)hipHostFree(h_steps);
+            if (d_steps) (void)hipFree(d_steps);
+            h_steps = nullptr;
+            d_steps = nullptr;
+            steps_cap = std::max(nj, 2 * steps_cap);
+            HIP_TRY(hipHostMalloc((void**)&h_steps, sizeof(JobStep) * RING * steps_cap, hipHostMallocDefault));
+            HIP_TRY(hipMalloc((void**)&d_steps, sizeof(JobStep) * RING * steps_cap));
+        }
+        if (nj > 0) HIP_TRY(hipMemcpy(d_jobs, desc.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice));
+        table_dirty = false;
+        return SADH_OK;
+    }
+
+    int flip(const std::vector<uint32_t>& which, hipStream_t s)
+    {
+        if (which.empty()) return SADH_OK;
+        std::vector<uint32_t> keep(which.size(), 0xFFFFFFFFu), one(which.size(), 1u);
+        return launch_update_jobs(d_jobs, which.data(), keep.data(), one.data(), (uint32_t)which.size(), s);
+    }
+
+    /* one iteration of every running fit; `done` receives the fits that finished */
+    int tick(std::vector<std::unique_ptr<FitRun>>& done)
+    {
+        if (fits.empty()) return SADH_OK;
+        if (table_dirty) SAD_TRY(rebuild_table());
+        hipStream_t S = sp.main;
+        const int nj = (int)fits.size();
+        const int passes = std::max(1, o.cand_update_passes);
+        int eff_ps = o.prune_start;
+        if (o.densify_enabled && !o.prune_during_densify && o.prune_start < o.densify_end) eff_ps = o.densify_end;
+        /* per-fit flags and step table (a ring slot whose previous tick has finished) */
+        const int slot = (int)(tick_no % RING);
+        HIP_TRY(hipEventSynchronize(slot_done[slot]));
+        JobStep* hs = h_steps + (size_t)slot * steps_cap;
+        JobStep* ds = d_steps + (size_t)slot * steps_cap;
+        bool any_upd = false, any_dens_upd = false, any_prune = false, any_log = false;
+        for (int j = 0; j < nj; ++j) {
+            FitRun& J = *fits[j];
+            const int it = J.iter;
+            const int eff_pe = o.prune_end > 0 ? o.prune_end : J.iterations - 1;
+            const int desired = std::max(0, (int)((float)J.active_est * J.d_pct));
+            J.upd = o.cand_update_freq > 0 && it % o.cand_update_freq == 0;
+            J.dens = o.densify_enabled && it >= o.densify_start && it <= o.densify_end &&
+                     it % std::max(1, o.densify_freq) == 0 && J.actual < J.cap && desired > 0;
+            J.prune = J.p_pct > 0.0f && J.pairs_count > 0 && it >= eff_ps && it < eff_pe &&
+                      it % std::max(1, o.prune_freq) == 0;
+            J.log = it % std::max(1, o.log_freq) == 0 || it == J.iterations - 1;
+            any_upd = any_upd || J.upd;
+            any_dens_upd = any_dens_upd || (J.upd && J.dens);
+            any_prune = any_prune || J.prune;
+            any_log = any_log || J.log;
+            const float blend = (float)it / (float)std::max(1, J.iterations);
+            const uint32_t t = (uint32_t)(it + 1);
+            hs[j].lambda = 0.05f * (0.1f + 0.9f * blend);
+            hs[j].b1c = 1.0f / (1.0f - powf(o.beta1, (float)t));
+            hs[j].b2c = 1.0f / (1.0f - powf(o.beta2, (float)t));
+            hs[j].jump_index = J.jump_index;
+            hs[j].vpt = J.upd ? 1u : 0u;
+        }
+        HIP_TRY(hipMemcpyAsync(ds, hs, sizeof(JobStep) * nj, hipMemcpyHostToDevice, S));
+        HIP_TRY(hipEventRecord(steps_ready, S));
+
+        std::vector<uint32_t> late;
         bool pending = false;
-        late_flip.clear();
-        if (passes > 0) {
-            hipStream_t U = any_dens ? S : sp.cand;
-            if (!any_dens) HIP_TRY(hipStreamWaitEvent(sp.cand, sp.sites_ready, 0));
-            for (int j = 0; j < n_jobs; ++j) {
-                JobRun& J = *jobs[j];
+        if (any_upd) {
+            /* an updating fit that densifies needs its new field first: then all updates run on S */
+            hipStream_t U = any_dens_upd ? S : sp.cand;
+            if (!any_dens_upd) HIP_TRY(hipStreamWaitEvent(sp.cand, steps_ready, 0));
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                if (!J.upd) continue;
                 J.n_upd++;
                 if (use_hil && (!J.hil_ready || J.hil_count != J.n)) {
                     int bits = sad_hilbert_bits(J.width, J.height);
@@ -280,34 +396,35 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
             }
             for (int p = 0; p < passes; ++p) {
                 BatchStep stp = {};
-                stp.jump_index = jump_index;
-                stp.vpt_parity = (uint32_t)(p & 1);
+                stp.vpt_pass = (uint32_t)p;
                 stp.radius_scale = o.cand_radius_scale;
                 stp.radius_probes = o.cand_radius_probes;
                 stp.inject_count = o.cand_inject_count;
                 stp.hil_probes = use_hil ? o.cand_hilbert_probes : 0u;
                 stp.hil_window = use_hil ? o.cand_hilbert_window : 0u;
-                SAD_TRY(launch_vpt_batch(d_jobs, (uint32_t)n_jobs, total_ctiles, total_sites, stp, U));
-                jump_index++;
+                SAD_TRY(launch_vpt_batch(d_jobs, (uint32_t)nj, ds, total_ctiles, total_sites, stp, U));
             }
-            /* an odd number of passes leaves the result in each job's other field: densifying jobs
-             * switch to it now, the others after their gradient (which reads the field before the update) */
-            if (passes & 1) {
-                std::vector<uint32_t> now;
-                for (int j = 0; j < n_jobs; ++j) ((any_dens && dens[j]) ? now : late_flip).push_back((uint32_t)j);
-                SAD_TRY(flip_jobs(d_jobs, now, S));
-                for (uint32_t j : now) jobs[j]->cur ^= 1u;
+            /* an odd pass count leaves the result in the other field: densifying fits switch now,
+             * the others after their gradient, which reads the field from before the update */
+            std::vector<uint32_t> now_flip;
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                if (!J.upd) continue;
+                J.jump_index += (uint32_t)passes;
+                if (passes & 1) (J.dens ? now_flip : late).push_back((uint32_t)j);
             }
-            if (!any_dens) {
+            SAD_TRY(flip(now_flip, S));
+            for (uint32_t j : now_flip) fits[j]->cur ^= 1u;
+            if (!any_dens_upd) {
                 HIP_TRY(hipEventRecord(sp.cand_done, sp.cand));
                 pending = true;
             }
         }
-        /* densify (per job, rare) */
-        upd_idx.clear(); upd_n.clear(); upd_flip.clear();
-        for (int j = 0; j < n_jobs; ++j) {
-            JobRun& J = *jobs[j];
-            if (!dens[j] || J.pairs_count <= 0) continue;
+        /* densify (per fit) */
+        std::vector<uint32_t> upd_idx, upd_n, upd_flip;
+        for (int j = 0; j < nj; ++j) {
+            FitRun& J = *fits[j];
+            if (!J.dens || J.pairs_count <= 0) continue;
             J.n_dens++;
             const int desired = std::max(0, (int)((float)J.active_est * J.d_pct));
             SAD_TRY(launch_clear(J.st, 8, J.n, S));
@@ -334,7 +451,7 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
             SAD_TRY(launch_update_jobs(d_jobs, upd_idx.data(), upd_n.data(), upd_flip.data(), (uint32_t)upd_idx.size(),
                                        S));
         if (any_prune)
-            for (int j = 0; j < n_jobs; ++j) SAD_TRY(launch_clear(&jobs[j]->removal, 1, jobs[j]->n, S));
+            for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear(&fits[j]->removal, 1, fits[j]->n, S));
 
         if (o.profile_gradients) {
             hipEvent_t a, b;
@@ -344,33 +461,27 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
             gev.push_back(b);
             HIP_TRY(hipEventRecord(a, S));
         }
-        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)n_jobs, total_tiles, any_prune, S));
+        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, any_prune, S));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             double b = 0.0;
-            for (int j = 0; j < n_jobs; ++j)
-                b += 48.0 * jobs[j]->np + 120.0 * jobs[j]->n + (any_prune ? 8.0 * jobs[j]->n : 0.0);
+            for (int j = 0; j < nj; ++j)
+                b += 48.0 * fits[j]->np + 120.0 * fits[j]->n + (any_prune ? 8.0 * fits[j]->n : 0.0);
             gbytes.push_back(b);
         }
         if (pending) HIP_TRY(hipStreamWaitEvent(S, sp.cand_done, 0));
-        if (!late_flip.empty()) {
-            SAD_TRY(flip_jobs(d_jobs, late_flip, S));
-            for (uint32_t j : late_flip) jobs[j]->cur ^= 1u;
-        }
-        BatchStep st = {};
-        const float blend = (float)iter / (float)std::max(1, o.iterations);
-        st.lambda = 0.05f * (0.1f + 0.9f * blend);
-        st.lr_pos = lr_pos; st.lr_tau = lr_tau; st.lr_radius = lr_rad; st.lr_color = lr_col;
-        st.lr_dir = lr_dir; st.lr_aniso = lr_an;
-        st.beta1 = o.beta1; st.beta2 = o.beta2; st.eps = o.eps;
-        const uint32_t t = (uint32_t)(iter + 1);
-        st.b1c = 1.0f / (1.0f - powf(o.beta1, (float)t));
-        st.b2c = 1.0f / (1.0f - powf(o.beta2, (float)t));
-        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)n_jobs, total_sites, st, S));
+        SAD_TRY(flip(late, S));
+        for (uint32_t j : late) fits[j]->cur ^= 1u;
 
-        for (int j = 0; j < n_jobs; ++j) {
-            JobRun& J = *jobs[j];
-            if (!prune[j]) continue;
+        BatchStep st = {};
+        st.lr_pos = o.lr_pos * o.lr_scale; st.lr_tau = o.lr_tau * o.lr_scale; st.lr_radius = o.lr_radius * o.lr_scale;
+        st.lr_color = o.lr_color * o.lr_scale; st.lr_dir = o.lr_dir * o.lr_scale; st.lr_aniso = o.lr_aniso * o.lr_scale;
+        st.beta1 = o.beta1; st.beta2 = o.beta2; st.eps = o.eps;
+        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, st, S));
+
+        for (int j = 0; j < nj; ++j) {
+            FitRun& J = *fits[j];
+            if (!J.prune) continue;
             J.n_prune++;
             SAD_TRY(launch_prune_pairs(J.d_sites, J.removal, J.pairs, J.n, J.delta_norm, (uint32_t)J.pairs_count, S));
             SAD_TRY(launch_radix_sort(J.pairs, J.sort_scratch, J.sort_hist, std::min(J.n, (uint32_t)J.pairs_count),
@@ -382,50 +493,57 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
                 J.active_est = std::max(0, J.active_est - k);
             }
         }
-        HIP_TRY(hipEventRecord(sp.sites_ready, S));
+        HIP_TRY(hipEventRecord(slot_done[slot], S));
+        tick_no++;
 
-        if (iter % std::max(1, o.log_freq) == 0 || iter == o.iterations - 1) {
-            std::vector<float> sse(n_jobs);
-            std::vector<uint32_t> cnt(n_jobs);
-            for (int j = 0; j < n_jobs; ++j) {
-                JobRun& J = *jobs[j];
-                SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw, J.ch,
-                                      S));
+        if (any_log) {
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                if (!J.log) continue;
+                SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
+                                      J.ch, S));
                 SAD_TRY(launch_sse(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
                 SAD_TRY(launch_count_active(J.d_sites, J.d_count, J.n, S));
-                HIP_TRY(hipMemcpyAsync(&sse[j], J.d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
-                HIP_TRY(hipMemcpyAsync(&cnt[j], J.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
+                HIP_TRY(hipMemcpyAsync(&J.sse, J.d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
+                HIP_TRY(hipMemcpyAsync(&J.active_count, J.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             }
             HIP_TRY(hipStreamSynchronize(S));
-            for (int j = 0; j < n_jobs; ++j) {
-                JobRun& J = *jobs[j];
-                float mse = sse[j] / (std::max(1.0f, J.mask_sum) * 3.0f);
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                if (!J.log) continue;
+                float mse = J.sse / (std::max(1.0f, J.mask_sum) * 3.0f);
                 float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
-                if (J.io->log_psnr && J.n_logs < J.io->log_cap) J.io->log_psnr[J.n_logs] = psnr;
+                if (J.io.log_psnr && J.n_logs < J.io.log_cap) J.io.log_psnr[J.n_logs] = psnr;
                 J.n_logs++;
                 J.final_psnr = psnr;
                 J.best = std::max(J.best, psnr);
-                J.active_count = cnt[j];
             }
         }
+        /* advance; retire finished fits */
+        bool finished = false;
+        for (auto& f : fits) {
+            f->iter++;
+            finished = finished || f->iter >= f->iterations;
+        }
+        if (finished) {
+            HIP_TRY(hipStreamSynchronize(S));
+            HIP_TRY(hipStreamSynchronize(sp.cand));
+            std::vector<std::unique_ptr<FitRun>> keep;
+            for (auto& f : fits) {
+                if (f->iter < f->iterations) { keep.push_back(std::move(f)); continue; }
+                HIP_TRY(hipMemcpy(f->io.sites_io, f->d_sites, sizeof(SadSite) * f->cap, hipMemcpyDeviceToHost));
+                done.push_back(std::move(f));
+            }
+            fits.swap(keep);
+            table_dirty = true;
+        }
+        return SADH_OK;
     }
-    HIP_TRY(hipStreamSynchronize(S));
-    HIP_TRY(hipStreamSynchronize(sp.cand));
-    auto t1 = std::chrono::steady_clock::now();
 
-    double gms = 0.0, gb = 0.0;
-    for (size_t i = 0; i + 1 < gev.size(); i += 2) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, gev[i], gev[i + 1]));
-        gms += ms;
-    }
-    for (double b : gbytes) gb += b;
-    const double secs = std::chrono::duration<double>(t1 - t0).count();
-    for (int j = 0; j < n_jobs; ++j) {
-        JobRun& J = *jobs[j];
-        HIP_TRY(hipMemcpy(J.io->sites_io, J.d_sites, sizeof(SadSite) * J.cap, hipMemcpyDeviceToHost));
-        SadhFitResult* r = J.io->result;
-        if (!r) continue;
+    void fill_result(const FitRun& J, double secs) const
+    {
+        SadhFitResult* r = J.io.result;
+        if (!r) return;
         r->final_psnr = J.final_psnr;
         r->best_psnr = J.best;
         r->actual_sites = J.actual;
@@ -443,23 +561,366 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
         r->grad_launches = 0;
         r->grad_bytes = 0.0;
     }
+
+    int grad_totals(double* ms, int* launches, double* bytes)
+    {
+        double g = 0.0, b = 0.0;
+        for (size_t i = 0; i + 1 < gev.size(); i += 2) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, gev[i], gev[i + 1]));
+            g += t;
+        }
+        for (double x : gbytes) b += x;
+        *ms = g;
+        *launches = (int)(gev.size() / 2);
+        *bytes = b;
+        return SADH_OK;
+    }
+};
+
+}  // namespace
+
+static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs, SadhBatchResult* bres)
+{
+    if (n_jobs <= 0 || !ios) return fail(SADH_ERR_INVALID, "fit_batch: no jobs");
+    Engine E;
+    SAD_TRY(E.init(o));
+    for (int j = 0; j < n_jobs; ++j) SAD_TRY(E.add_fit(ios[j], j, 0.0));
+    std::vector<std::unique_ptr<FitRun>> done;
+    auto t0 = Clock::now();
+    while (!E.fits.empty()) SAD_TRY(E.tick(done));
+    HIP_TRY(hipStreamSynchronize(E.sp.main));
+    HIP_TRY(hipStreamSynchronize(E.sp.cand));
+    const double secs = std::chrono::duration<double>(Clock::now() - t0).count();
+    double px = 0.0;
+    for (auto& f : done) {
+        E.fill_result(*f, secs);
+        px += (double)f->np;
+    }
     if (bres) {
         bres->train_seconds = secs;
-        bres->grad_kernel_ms = gms;
-        bres->grad_launches = (int)(gev.size() / 2);
-        bres->grad_bytes = gb;
-        bres->total_pixels = 0.0;
-        for (int j = 0; j < n_jobs; ++j) bres->total_pixels += (double)jobs[j]->np;
+        SAD_TRY(E.grad_totals(&bres->grad_kernel_ms, &bres->grad_launches, &bres->grad_bytes));
+        bres->total_pixels = px;
     }
     return SADH_OK;
 }
+
+/* ---------------------------------------------------------------- server -- */
+
+namespace {
+
+struct RenderRun {
+    long long id = 0;
+    const SadSite* sites_h = nullptr;
+    uint32_t n = 0;
+    int width = 0, height = 0;
+    float* out_h = nullptr;
+    size_t bytes = 0;
+    double submit_s = 0.0, admit_s = 0.0;
+    DevArena mem;
+    hipEvent_t done = nullptr;
+    ~RenderRun()
+    {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+struct Request {
+    int kind = 0;            /* 0 fit, 1 render */
+    long long id = 0;
+    double submit_s = 0.0;
+    size_t bytes = 0;
+    SadhBatchJob fit = {};
+    const SadSite* sites = nullptr;
+    uint32_t n = 0;
+    int width = 0, height = 0;
+    float* out = nullptr;
+};
+
+}  // namespace
+
+struct SadhServerState {
+    SadhServerOptions so = {};
+    Engine E;
+    hipStream_t rstream = nullptr;
+    std::deque<Request> queue;
+    std::vector<std::unique_ptr<RenderRun>> renders;
+    std::vector<SadhServerEvent> events;
+    size_t budget = 0, in_use = 0;
+    std::vector<size_t> fit_bytes_of;   /* parallel to E.fits */
+    long long next_id = 1;
+    bool use_hil_render = true;
+
+    ~SadhServerState()
+    {
+        if (E.sp.main) (void)hipStreamSynchronize(E.sp.main);
+        if (E.sp.cand) (void)hipStreamSynchronize(E.sp.cand);
+        if (rstream) (void)hipStreamSynchronize(rstream);
+        renders.clear();
+        if (rstream) (void)hipStreamDestroy(rstream);
+    }
+
+    int render_start(RenderRun& R)
+    {
+        const int w = R.width, h = R.height;
+        const uint32_t n = R.n;
+        const int passes = so.render_passes > 0 ? so.render_passes : 16;
+        const bool use_hil = use_hil_render;
+        const size_t cells = (size_t)w * h;
+        SadSite* d_sites;
+        SadSearchSite* search;
+        uint32_t *a0, *a1, *b0, *b1;
+        float* d_out;
+        SAD_TRY(R.mem.alloc(&d_sites, n));
+        SAD_TRY(R.mem.alloc(&search, n));
+        SAD_TRY(R.mem.alloc(&a0, cells * 4));
+        SAD_TRY(R.mem.alloc(&a1, cells * 4));
+        SAD_TRY(R.mem.alloc(&b0, cells * 4));
+        SAD_TRY(R.mem.alloc(&b1, cells * 4));
+        SAD_TRY(R.mem.alloc(&d_out, cells * 3));
+        const uint32_t padded = div_up(n, 1024u) * 1024u;
+        uint32_t *h_order = nullptr, *h_pos = nullptr, *h_hist = nullptr;
+        SadPair *h_pairs = nullptr, *h_scratch = nullptr;
+        if (use_hil) {
+            SAD_TRY(R.mem.alloc(&h_order, n));
+            SAD_TRY(R.mem.alloc(&h_pos, n));
+            SAD_TRY(R.mem.alloc(&h_pairs, padded));
+            SAD_TRY(R.mem.alloc(&h_scratch, padded));
+            SAD_TRY(R.mem.alloc(&h_hist, 256u * (padded / 1024u)));
+        }
+        hipStream_t S = rstream;
+        HIP_TRY(hipMemcpyAsync(d_sites, R.sites_h, sizeof(SadSite) * n, hipMemcpyHostToDevice, S));
+        const int md = std::max(w, h);
+        const float iss = 1.0f / ((float)md * (float)md);
+        if (use_hil) {
+            int bits = sad_hilbert_bits(w, h);
+            uint32_t mk = bits >= 16 ? 0xFFFFFFFFu : (1u << (2 * bits));
+            SAD_TRY(launch_hilbert_pairs(d_sites, h_pairs, n, padded, w, h, bits, S));
+            SAD_TRY(launch_radix_sort(h_pairs, h_scratch, h_hist, padded, mk, S));
+            SAD_TRY(launch_hilbert_order(h_pairs, h_order, h_pos, n, S));
+        }
+        /* render-job recipe (sad_cuda.cu:1616-1655, 1678-1858): clear, seed, JFA flood, VPT passes, render;
+         * the same launches as sadh_render_sites with one JFA round */
+        SAD_TRY(sadh_clear_candidates(a0, a1, w, h, (sadh_stream_t)S));
+        SAD_TRY(sadh_jfa_seed(a0, d_sites, n, w, h, 1, (sadh_stream_t)S));
+        SAD_TRY(launch_search_sites(d_sites, search, n, iss, false, S));
+        int step = 1;
+        while (step < std::max(w, h)) step <<= 1;
+        uint32_t *in = a0, *outp = a1;
+        for (int s = step / 2; s >= 1; s /= 2) {
+            SAD_TRY(launch_jfa_flood_search(in, outp, search, n, (uint32_t)s, iss, w, h, 1, w, h, S));
+            std::swap(in, outp);
+        }
+        if (in != a0) HIP_TRY(hipMemcpyAsync(a0, in, cells * 16, hipMemcpyDeviceToDevice, S));
+        SAD_TRY(launch_search_sites(d_sites, search, n, iss, true, S));
+        uint32_t jump_index = 0;
+        for (int p = 0; p < passes; ++p) {
+            uint32_t stp = sad_pack_jump_step(jump_index, w, h);
+            SAD_TRY(launch_vpt_search(a0, a1, b0, b1, search, n, stp, iss, jump_index >> 16, so.fit.cand_radius_scale,
+                                      so.fit.cand_radius_probes, so.fit.cand_inject_count, h_order, h_pos,
+                                      use_hil ? so.fit.cand_hilbert_probes : 0u,
+                                      use_hil ? so.fit.cand_hilbert_window : 0u, 1, w, h, w, h, S));
+            jump_index++;
+            std::swap(a0, b0);
+            std::swap(a1, b1);
+        }
+        SAD_TRY(launch_render(a0, a1, d_out, d_sites, iss, n, w, h, w, h, S));
+        HIP_TRY(hipMemcpyAsync(R.out_h, d_out, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, S));
+        HIP_TRY(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(R.done, S));
+        return SADH_OK;
+    }
+
+    int admit()
+    {
+        const int max_fits = so.max_fit_jobs > 0 ? so.max_fit_jobs : 64;
+        while (!queue.empty()) {
+            const Request& q = queue.front();
+            if (in_use + q.bytes > budget) break;   /* arrival order: a request waits for the one before it */
+            if (q.kind == 0 && (int)E.fits.size() >= max_fits) break;
+            Request r = q;
+            queue.pop_front();
+            if (r.kind == 0) {
+                SAD_TRY(E.add_fit(r.fit, r.id, r.submit_s));
+                fit_bytes_of.push_back(r.bytes);
+            } else {
+                std::unique_ptr<RenderRun> R(new RenderRun());
+                R->id = r.id;
+                R->sites_h = r.sites;
+                R->n = r.n;
+                R->width = r.width;
+                R->height = r.height;
+                R->out_h = r.out;
+                R->bytes = r.bytes;
+                R->submit_s = r.submit_s;
+                R->admit_s = E.now();
+                SAD_TRY(render_start(*R));
+                renders.push_back(std::move(R));
+            }
+            in_use += r.bytes;
+        }
+        return SADH_OK;
+    }
+
+    int retire_renders()
+    {
+        std::vector<std::unique_ptr<RenderRun>> keep;
+        for (auto& R : renders) {
+            hipError_t e = hipEventQuery(R->done);
+            if (e == hipErrorNotReady) { keep.push_back(std::move(R)); continue; }
+            if (e != hipSuccess) return fail((int)e, std::string("server render: ") + hipGetErrorString(e));
+            SadhServerEvent ev = {};
+            ev.id = R->id;
+            ev.kind = 1;
+            ev.submit_s = R->submit_s;
+            ev.admit_s = R->admit_s;
+            ev.done_s = E.now();
+            events.push_back(ev);
+            in_use -= R->bytes;
+        }
+        renders.swap(keep);
+        return SADH_OK;
+    }
+
+    int step(int* outstanding)
+    {
+        SAD_TRY(retire_renders());
+        SAD_TRY(admit());
+        if (!E.fits.empty()) {
+            std::vector<long long> ids_before;
+            for (auto& f : E.fits) ids_before.push_back(f->id);
+            std::vector<std::unique_ptr<FitRun>> done;
+            SAD_TRY(E.tick(done));
+            if (!done.empty()) {
+                const double t = E.now();
+                std::vector<size_t> kept;
+                for (size_t i = 0; i < ids_before.size(); ++i) {
+                    bool gone = false;
+                    for (auto& f : done) gone = gone || f->id == ids_before[i];
+                    if (gone) in_use -= fit_bytes_of[i];
+                    else kept.push_back(fit_bytes_of[i]);
+                }
+                fit_bytes_of.swap(kept);
+                for (auto& f : done) {
+                    E.fill_result(*f, t - f->admit_s);
+                    SadhServerEvent ev = {};
+                    ev.id = f->id;
+                    ev.kind = 0;
+                    ev.submit_s = f->submit_s;
+                    ev.admit_s = f->admit_s;
+                    ev.done_s = t;
+                    events.push_back(ev);
+                }
+            }
+        } else if (!renders.empty()) {
+            /* no fit to advance: wait for the oldest render rather than spin */
+            HIP_TRY(hipEventSynchronize(renders.front()->done));
+        }
+        SAD_TRY(retire_renders());
+        if (outstanding) *outstanding = (int)(E.fits.size() + renders.size() + queue.size());
+        return SADH_OK;
+    }
+};
 
 }  // namespace sadh
 
 using namespace sadh;
 
-extern "C" int sadh_fit_batch(const SadhFitOptions* opts, SadhBatchJob* jobs, int n_jobs, SadhBatchResult* result)
+struct SadhServer : SadhServerState {};
+
+extern "C" {
+
+int sadh_fit_batch(const SadhFitOptions* opts, SadhBatchJob* jobs, int n_jobs, SadhBatchResult* result)
 {
     SADH_REQUIRE(opts && jobs, "fit_batch: null pointer");
     return fit_batch_impl(*opts, jobs, n_jobs, result);
 }
+
+int sadh_server_create(const SadhServerOptions* opts, sadh_server_t* out)
+{
+    SADH_REQUIRE(opts && out, "server_create: null pointer");
+    std::unique_ptr<SadhServer> S(new SadhServer());
+    S->so = *opts;
+    SAD_TRY(S->E.init(opts->fit));
+    S->use_hil_render = opts->fit.cand_hilbert_probes > 0 && opts->fit.cand_hilbert_window > 0;
+    HIP_TRY(hipStreamCreateWithFlags(&S->rstream, hipStreamNonBlocking));
+    if (opts->memory_budget > 0.0) {
+        S->budget = (size_t)opts->memory_budget;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        S->budget = (size_t)(0.8 * (double)total_b);
+    }
+    *out = S.release();
+    return SADH_OK;
+}
+
+int sadh_server_submit_fit(sadh_server_t server, const SadhBatchJob* job, long long* id)
+{
+    SADH_REQUIRE(server && job && id, "server_submit_fit: null pointer");
+    SADH_REQUIRE(job->target && job->sites_io && job->width > 0 && job->height > 0 && job->n_sites > 0,
+                 "server_submit_fit: bad job");
+    float dp, pp;
+    int cap, pairs;
+    SAD_TRY(fit_plan_for(server->so.fit, *job, &dp, &pp, &cap, &pairs));
+    if (job->sites_cap < cap)
+        return fail(SADH_ERR_CAPACITY, "server_submit_fit: sites_io holds " + std::to_string(job->sites_cap) +
+                                           " sites, the plan needs " + std::to_string(cap));
+    Request r;
+    r.kind = 0;
+    r.id = server->next_id++;
+    r.submit_s = server->E.now();
+    r.fit = *job;
+    r.bytes = fit_bytes(server->so.fit, job->width, job->height, cap, pairs, server->E.use_hil);
+    if (r.bytes > server->budget) return fail(SADH_ERR_CAPACITY, "server_submit_fit: job exceeds the memory budget");
+    server->queue.push_back(r);
+    *id = r.id;
+    return SADH_OK;
+}
+
+int sadh_server_submit_render(sadh_server_t server, const SadSite* sites, uint32_t site_count, int width, int height,
+                              float* output, long long* id)
+{
+    SADH_REQUIRE(server && sites && output && id && site_count > 0 && width > 0 && height > 0,
+                 "server_submit_render: bad arguments");
+    Request r;
+    r.kind = 1;
+    r.id = server->next_id++;
+    r.submit_s = server->E.now();
+    r.sites = sites;
+    r.n = site_count;
+    r.width = width;
+    r.height = height;
+    r.out = output;
+    r.bytes = render_bytes(site_count, width, height, server->use_hil_render);
+    if (r.bytes > server->budget) return fail(SADH_ERR_CAPACITY, "server_submit_render: job exceeds the memory budget");
+    server->queue.push_back(r);
+    *id = r.id;
+    return SADH_OK;
+}
+
+int sadh_server_step(sadh_server_t server, int* outstanding)
+{
+    SADH_REQUIRE(server, "server_step: null server");
+    return server->step(outstanding);
+}
+
+int sadh_server_poll(sadh_server_t server, SadhServerEvent* events, int cap, int* count)
+{
+    SADH_REQUIRE(server && count && (cap == 0 || events), "server_poll: null pointer");
+    int k = std::min(cap, (int)server->events.size());
+    for (int i = 0; i < k; ++i) events[i] = server->events[i];
+    server->events.erase(server->events.begin(), server->events.begin() + k);
+    *count = k;
+    return SADH_OK;
+}
+
+double sadh_server_time(sadh_server_t server) { return server ? server->E.now() : 0.0; }
+
+int sadh_server_destroy(sadh_server_t server)
+{
+    delete server;
+    return SADH_OK;
+}
+
+}  // extern "C"

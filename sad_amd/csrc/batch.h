@@ -62,13 +62,19 @@ __device__ __forceinline__ uint32_t batch_find(const BatchJob* __restrict__ jobs
     return lo;
 }
 
-/* iteration scalars shared by every job of a batched launch */
+/* per-job scalars of one engine tick (jobs may be at different iterations) */
+struct JobStep {
+    float lambda;              /* tau diffusion weight at the job's iteration */
+    float b1c, b2c;            /* Adam bias corrections 1/(1-beta^t), t = iteration + 1 */
+    uint32_t jump_index;       /* the job's VPT pass index */
+    uint32_t vpt;              /* 1: the job updates its candidates this tick */
+};
+
+/* scalars shared by every job of a batched launch */
 struct BatchStep {
-    float lambda;              /* tau diffusion weight */
     float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
-    float beta1, beta2, eps, b1c, b2c;
-    uint32_t jump_index;       /* VPT pass index (the jump step depends on each job's grid) */
-    uint32_t vpt_parity;       /* VPT pass p of an update reads field cur ^ (p & 1), writes the other */
+    float beta1, beta2, eps;
+    uint32_t vpt_pass;         /* pass p of this tick's update: jump index + p, reads field cur ^ (p & 1) */
     float radius_scale;
     uint32_t radius_probes, inject_count, hil_probes, hil_window;
 };

@@ -103,8 +103,8 @@ int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*
 
 /* batched fit (one launch over every job) */
 int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);
-int launch_vpt_batch(const BatchJob*, uint32_t, uint32_t, uint32_t, const BatchStep&, hipStream_t);
-int launch_tau_adam_batch(const BatchJob*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
+int launch_vpt_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
+int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, const BatchStep&, hipStream_t);
 int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
 
 /* capacity and densify/prune percentiles of a fit (sad_cuda.cu:904-1098) */

@@ -347,32 +347,37 @@ __global__ __launch_bounds__(256) void k_vpt(const uint4* __restrict__ in0, cons
 /* Batched fit: one VPT pass over the concatenated candidate tiles of every job
  * (each job's current field -> its other field); the jump step comes from each
  * job's grid (packJumpStep, sad_cuda.cu:1157-1183). */
-__global__ __launch_bounds__(256) void k_vpt_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, BatchStep st)
+__global__ __launch_bounds__(256) void k_vpt_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                   const JobStep* __restrict__ steps, BatchStep st)
 {
     const uint32_t f = blockIdx.x;
     const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::ctile_begin);
+    if (!steps[j].vpt) return;
+    const uint32_t jump_index = steps[j].jump_index + st.vpt_pass;
     const BatchJob& J = jobs[j];
     const uint32_t t = f - J.ctile_begin;
     const int x = (int)(t % J.ctiles_x) * 16 + (int)(threadIdx.x & 15u);
     const int y = (int)(t / J.ctiles_x) * 16 + (int)(threadIdx.x >> 4);
     if (x >= J.cw || y >= J.ch) return;
-    const uint32_t src = J.cur ^ st.vpt_parity;
+    const uint32_t src = J.cur ^ (st.vpt_pass & 1u);
     const uint4* in0 = reinterpret_cast<const uint4*>(src ? J.fb0 : J.fa0);
     const uint4* in1 = reinterpret_cast<const uint4*>(src ? J.fb1 : J.fa1);
     uint4* out0 = reinterpret_cast<uint4*>(src ? J.fa0 : J.fb0);
     uint4* out1 = reinterpret_cast<uint4*>(src ? J.fa1 : J.fb1);
     SearchFetch fetch{reinterpret_cast<const float4*>(J.search)};
     const bool hil = J.h_order != nullptr && st.hil_probes > 0u && st.hil_window > 0u;
-    vpt_cell(x, y, in0, in1, out0, out1, fetch, J.n, sad_pack_jump_step(st.jump_index, J.cw, J.ch), J.iss,
-             st.jump_index >> 16, st.radius_scale, st.radius_probes, st.inject_count, J.h_order, J.h_pos,
+    vpt_cell(x, y, in0, in1, out0, out1, fetch, J.n, sad_pack_jump_step(jump_index, J.cw, J.ch), J.iss,
+             jump_index >> 16, st.radius_scale, st.radius_probes, st.inject_count, J.h_order, J.h_pos,
              hil ? st.hil_probes : 0u, hil ? st.hil_window : 0u, J.ds, J.width, J.height, J.cw, J.ch);
 }
 
 /* Batched fit: search sites (fp16-rounded key terms) of every job's sites. */
-__global__ __launch_bounds__(256) void k_search_batch(const BatchJob* __restrict__ jobs, uint32_t njobs)
+__global__ __launch_bounds__(256) void k_search_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                      const JobStep* __restrict__ steps)
 {
     const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::site_begin);
+    if (!steps[j].vpt) return;
     const BatchJob& J = jobs[j];
     const uint32_t i = f - J.site_begin;
     if (i >= J.n) return;
@@ -405,12 +410,15 @@ int launch_vpt_search(const uint32_t* in0, const uint32_t* in1, uint32_t* out0, 
     return launch_status("update_candidates(search)");
 }
 
-int launch_vpt_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_ctiles, uint32_t total_sites,
-                     const BatchStep& st, hipStream_t stream)
+/* VPT pass `st.vpt_pass` of this tick's update for the jobs with steps[j].vpt
+ * (the first pass also prepares their search sites) */
+int launch_vpt_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total_ctiles,
+                     uint32_t total_sites, const BatchStep& st, hipStream_t stream)
 {
-    if (total_sites > 0)
-        hipLaunchKernelGGL(k_search_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs);
-    if (total_ctiles > 0) hipLaunchKernelGGL(k_vpt_batch, dim3(total_ctiles), dim3(256), 0, stream, jobs, njobs, st);
+    if (total_sites > 0 && st.vpt_pass == 0u)
+        hipLaunchKernelGGL(k_search_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs, steps);
+    if (total_ctiles > 0)
+        hipLaunchKernelGGL(k_vpt_batch, dim3(total_ctiles), dim3(256), 0, stream, jobs, njobs, steps, st);
     return launch_status("update_candidates_batch");
 }
 

@@ -238,23 +238,24 @@ __global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__
  * with Adam + clear, over the concatenated sites of every job; the candidate
  * field is each job's current one. */
 __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t pass,
-                                                   BatchStep st)
+                                                   const JobStep* __restrict__ steps, BatchStep st)
 {
     const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     const uint32_t jx = batch_find(jobs, njobs, f, &BatchJob::site_begin);
     const BatchJob& J = jobs[jx];
+    const JobStep js = steps[jx];
     const uint32_t i = f - J.site_begin;
     if (i >= J.n) return;
     const uint4* c0 = reinterpret_cast<const uint4*>(J.cur ? J.fb0 : J.fa0);
     const uint4* c1 = reinterpret_cast<const uint4*>(J.cur ? J.fb1 : J.fa1);
     const float inv_ds = 1.0f / fmaxf((float)J.ds, 1.0f);
     if (pass == 0u) {
-        tau_site(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, st.lambda, J.cw, J.ch, inv_ds,
+        tau_site(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, js.lambda, J.cw, J.ch, inv_ds,
                  16u);
     } else if (pass == 1u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, J.tau_tmp2, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, J.tau_tmp2, nullptr, J.n, js.lambda, J.cw, J.ch, inv_ds);
     } else if (pass == 2u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp2, J.tau_tmp, nullptr, J.n, st.lambda, J.cw, J.ch, inv_ds);
+        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp2, J.tau_tmp, nullptr, J.n, js.lambda, J.cw, J.ch, inv_ds);
     } else {
         AdamArgs A;
         A.sites = J.sites; A.st = J.adam;
@@ -263,8 +264,8 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
         A.gstride = 16u;
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
-        A.b1c = st.b1c; A.b2c = st.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
-        tau_last_adam_site(i, c0, c1, J.tau_raw, J.tau_tmp, st.lambda, J.cw, J.ch, inv_ds, A);
+        A.b1c = js.b1c; A.b2c = js.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
+        tau_last_adam_site(i, c0, c1, J.tau_raw, J.tau_tmp, js.lambda, J.cw, J.ch, inv_ds, A);
     }
 }
 
@@ -546,12 +547,13 @@ int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* t
     return launch_status("tau_diffuse_adam");
 }
 
-int launch_tau_adam_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_sites, const BatchStep& st,
-                          hipStream_t stream)
+int launch_tau_adam_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total_sites,
+                          const BatchStep& st, hipStream_t stream)
 {
     if (total_sites == 0) return SADH_OK;
     for (uint32_t pass = 0; pass < 4u; ++pass)
-        hipLaunchKernelGGL(k_tau_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs, pass, st);
+        hipLaunchKernelGGL(k_tau_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs, pass, steps,
+                           st);
     return launch_status("tau_diffuse_adam_batch");
 }
 

@@ -148,3 +148,32 @@ def batch_workload(n_images: int = 64, seed: int = 4, min_side: int = 128, max_s
         sites.append(init_sites_uniform(t, n, seed * 1000 + i))
     return targets, sites
 
+
+def serve_workload(n_requests: int = 64, rate: float = 16.0, seed: int = 5, fit_fraction: float = 0.7,
+                   fit_sides=(256, 2048), render_sides=(512, 4096), render_sites=(4096, 1 << 20)):
+    """The served fit/render stream: Poisson arrivals at `rate` per second; a
+    request is a fit with probability `fit_fraction` (side log-uniform in
+    fit_sides, aspect log-uniform in [0.5, 2], bpp-1.0 initial sites:
+    W*H/128) and otherwise a render of a random site set (site count and
+    output side log-uniform in render_sites / render_sides, square output).
+    Returns a list of dicts {t, kind, target|sites, width, height}."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < fit_fraction:
+            side = float(np.exp(rng.uniform(np.log(fit_sides[0]), np.log(fit_sides[1]))))
+            asp = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+            w = max(16, int(round(side * np.sqrt(asp))))
+            h = max(16, int(round(side / np.sqrt(asp))))
+            tgt = make_target(w, h, seed * 7919 + i, n_ellipses=40)
+            st = init_sites_uniform(tgt, max(16, int(round(w * h / 128.0))), seed * 7919 + i)
+            out.append(dict(t=t, kind="fit", target=tgt, sites=st, width=w, height=h))
+        else:
+            side = int(round(float(np.exp(rng.uniform(np.log(render_sides[0]), np.log(render_sides[1]))))))
+            n = int(round(float(np.exp(rng.uniform(np.log(render_sites[0]), np.log(render_sites[1]))))))
+            out.append(dict(t=t, kind="render", sites=init_sites_random(side, side, n, seed * 7919 + i),
+                            width=side, height=side))
+    return out
+
