@@ -24,6 +24,9 @@ Other workloads (not the headline line):
                          inputs; the line carries every kernel's row.
   --workload batch64     64 images (about 60 MP) fitted as one batch for 500
                          iterations, and one at a time on the same inputs.
+  --workload serve       a Poisson stream of fit and render requests replayed
+                         against the continuous-batching server: throughput
+                         and p50/p99 latency per request kind.
 """
 import argparse
 import json
@@ -167,12 +170,46 @@ def batch_workload(args) -> int:
     return 0
 
 
+def serve_workload(args) -> int:
+    """--workload serve: a Poisson stream (16 requests/s, 64 requests) of 70%
+    fits (sides 256-2048, bpp 1.0, 300 iterations) and 30% renders (4k-1M
+    random sites, 512^2-4096^2), replayed in wall-clock time against the
+    continuous-batching server; throughput and p50/p99 latency per kind."""
+    import torch
+
+    from sad_amd import config, serve, synth
+    torch.cuda.set_device(0)
+    reqs = synth.serve_workload(n_requests=args.requests, rate=args.rate)
+    opts = config.fit_options(iterations=300, target_bpp=1.0, log_freq=10**9)
+    warm = synth.serve_workload(n_requests=4, rate=1000.0, seed=99, fit_sides=(128, 256), render_sides=(256, 512),
+                                render_sites=(1024, 4096))
+    for _ in range(args.warmup):
+        serve.run_stream(warm, opts)
+    r = serve.run_stream(reqs, opts)
+    line = {
+        "metric": "fit megapixel-iters/sec (fwd+bwd+Adam)", "value": round(r["fit_mp_iters_per_s"], 3),
+        "unit": "MP-iters/s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["wall_s"], 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serve", "requests": len(reqs), "rate_per_s": args.rate, "fits": r["n_fit"],
+                   "renders": r["n_render"], "fit_iterations": 300},
+        "latency_s": {"fit_p50": r["fit_latency_p50_s"], "fit_p99": r["fit_latency_p99_s"],
+                      "render_p50": r["render_latency_p50_s"], "render_p99": r["render_latency_p99_s"]},
+        "render_mp_per_s": round(r["render_mp_per_s"], 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64"])
+    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64", "serve"])
+    ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate per second")
     ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -185,6 +222,8 @@ def main() -> int:
         return kernel_workload(args)
     if args.workload == "batch64":
         return batch_workload(args)
+    if args.workload == "serve":
+        return serve_workload(args)
 
     import torch
 
