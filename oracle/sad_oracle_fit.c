@@ -218,7 +218,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     float mask_sum = 0.0f;
     if (mask_in) {
         memcpy(mask, mask_in, sizeof(float) * np);
-        for (int i = 0; i < np; ++i) mask_sum += mask[i];
+        for (int i = 0; i < np; ++i) mask_sum += mask[i] > 0.0f ? 1.0f : 0.0f;   /* sad_cuda.cu:207-213 */
     } else {
         for (int i = 0; i < np; ++i) mask[i] = 1.0f;
         mask_sum = (float)np;

@@ -132,7 +132,8 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     J.mask_sum = (float)J.np;
     if (io.mask) {
         J.mask_sum = 0.0f;
-        for (int i = 0; i < J.np; ++i) { mask[i] = io.mask[i]; J.mask_sum += io.mask[i]; }
+        /* loadMask, sad_cuda.cu:207-213: the normaliser counts pixels with a positive mask */
+        for (int i = 0; i < J.np; ++i) { mask[i] = io.mask[i]; J.mask_sum += io.mask[i] > 0.0f ? 1.0f : 0.0f; }
     }
     J.delta_norm = 1.0f / std::max(1.0f, J.mask_sum);
     std::vector<SadSite> hs(J.cap);

@@ -152,7 +152,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     float mask_sum = (float)np;
     if (mask_h) {
         mask_sum = 0.0f;
-        for (int i = 0; i < np; ++i) { mask[i] = mask_h[i]; mask_sum += mask_h[i]; }
+        /* loadMask, sad_cuda.cu:207-213: the normaliser counts pixels with a positive mask */
+        for (int i = 0; i < np; ++i) { mask[i] = mask_h[i]; mask_sum += mask_h[i] > 0.0f ? 1.0f : 0.0f; }
     }
     std::vector<SadSite> host_sites(cap);
     for (int i = 0; i < cap; ++i) {
