@@ -233,11 +233,26 @@ int sadh_compute_psnr(const float* rendered, const float* target, const float* m
 int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count,
                             sadh_stream_t stream);
 
+/* launchInitGradientWeighted, densify_prune.cu:5-83,312-334: num_sites sites at
+ * the first accepted seeds of the R2 sequence (mask > 0 and mask-weighted Sobel
+ * magnitude of the luma > grad_threshold; the reference CLI passes 0.01 * alpha
+ * and 256 attempts).  *seed_counter (device, nullable = 0) is the first seed and
+ * receives one past the last seed drawn.  The reference's sites race for seeds
+ * on one atomic counter; here site i draws after site i-1, so sites come in
+ * seed order and the result is deterministic.  A site that draws max_attempts
+ * rejected seeds is inactive (pos -1, colour 0.5).  mask may be NULL (all ones). */
+int sadh_init_gradient_weighted(SadSite* sites, uint32_t num_sites, uint32_t* seed_counter,
+                                const float* target, const float* mask, float grad_threshold,
+                                uint32_t max_attempts, float init_log_tau, float init_radius,
+                                int width, int height, sadh_stream_t stream);
+
 /* ------------------------------------------------------- fit / render ---- */
 
 /* Training options: the TrainingOptions fields of sad_cuda.cu:528-576 that the
  * fit schedule reads.  init_mode: 0 = sites given by the caller
- * (--init-from-sites), 1 = one site per pixel (--init-per-pixel). */
+ * (--init-from-sites), 1 = one site per pixel (--init-per-pixel), 2 = n_sites
+ * gradient-weighted sites made on the device (--init-gradient, the reference
+ * CLI's default; sad_cuda.cu:2078-2095). */
 typedef struct SadhFitOptions {
     int iterations;
     float target_bpp;                 /* <= 0: keep the percentiles below */
@@ -257,6 +272,8 @@ typedef struct SadhFitOptions {
     float beta1, beta2, eps;
     int log_freq;
     int profile_gradients;            /* 1: time every gradient launch with HIP events */
+    float init_gradient_alpha;        /* init_mode 2: threshold = 0.01 * alpha */
+    float init_log_tau, init_radius;  /* init_mode 2: initial log_tau and radius */
 } SadhFitOptions;
 
 typedef struct SadhFitResult {
@@ -271,6 +288,8 @@ typedef struct SadhFitResult {
     double grad_bytes;                /* summed algorithmic bytes of those launches */
     float* grad_ms_trace;             /* in: optional host array; out: per-launch gradient ms */
     int grad_ms_trace_cap;            /* in: entries of grad_ms_trace */
+    float* final_render;              /* in: optional host [H][W][3]; out: render of the final sites with the
+                                         final candidates (sad_cuda.cu:2482-2496); sadh_fit only */
 } SadhFitResult;
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does

@@ -30,6 +30,7 @@ class OrcFitConfig(C.Structure):
         ("cand_hilbert_window", U32), ("cand_hilbert_probes", U32),
         ("lr_scale", F32), ("lr_pos", F32), ("lr_tau", F32), ("lr_radius", F32), ("lr_color", F32),
         ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32), ("log_freq", I32),
+        ("init_gradient_alpha", F32), ("init_log_tau", F32), ("init_radius", F32),
     ]
 
 
@@ -56,6 +57,8 @@ def lib() -> C.CDLL:
         _lib.orc_jump_step.restype = U32
         _lib.orc_get_threads.restype = I32
         _lib.orc_fit.restype = I32
+        _lib.orc_init_gradient_weighted.restype = U32
+        _lib.orc_init_grad_eval.restype = I32
     return _lib
 
 
@@ -265,15 +268,18 @@ def fit_plan(opts: dict, width: int, height: int) -> dict:
 
 
 def fit(target, sites, opts: dict, mask=None) -> dict:
+    """sites: initial sites [N][10], or None with init_mode 2 (gradient-weighted)."""
     h, w, _ = target.shape
     o = dict(opts)
     if o["init_mode"] == 0:
         o["n_sites"] = int(sites.shape[0])
     plan = fit_plan(o, w, h)
-    cap = max(plan["capacity"], sites.shape[0])
+    n_in = 0 if sites is None else sites.shape[0]
+    cap = max(plan["capacity"], n_in)
     buf = np.zeros((cap, 10), np.float32)
     buf[:, 0:2] = -1.0
-    buf[: sites.shape[0]] = sites
+    if n_in:
+        buf[:n_in] = sites
     t = _f32(target)
     m = None if mask is None else _f32(mask)
     logs = np.zeros(64, np.float32)
@@ -287,3 +293,24 @@ def fit(target, sites, opts: dict, mask=None) -> dict:
     out["log_psnr"] = logs[: min(res.n_logs, 64)].copy()
     out["plan"] = plan
     return out
+
+
+def init_gradient_weighted(target, n, mask=None, threshold=0.01, attempts=256, log_tau0=5.0, radius0=4.0, seed0=0):
+    """-> (sites [n][10], one past the last seed drawn)"""
+    t = _f32(target)
+    h, w, _ = t.shape
+    m = None if mask is None else _f32(mask)
+    out = np.zeros((n, 10), np.float32)
+    end = lib().orc_init_gradient_weighted(_p(out), U32(n), U32(seed0), _p(t), _p(m), F32(threshold), U32(attempts),
+                                           F32(log_tau0), F32(radius0), I32(w), I32(h))
+    return out, int(end)
+
+
+def init_grad_eval(target, seed, mask=None, threshold=0.01):
+    """-> (accepted, x, y, pixel index, gradient magnitude) of one R2 seed"""
+    t = _f32(target)
+    h, w, _ = t.shape
+    m = None if mask is None else _f32(mask)
+    o = np.zeros(4, np.float32)
+    ok = lib().orc_init_grad_eval(_p(t), _p(m), I32(w), I32(h), U32(seed), F32(threshold), _p(o))
+    return bool(ok), float(o[0]), float(o[1]), int(o[2]), float(o[3])

@@ -786,3 +786,39 @@ ORC_API double orc_sse(const float* rendered, const float* target, const float* 
     }
     return s;
 }
+
+/* initGradientWeightedKernel, densify_prune.cu:5-83.  The reference's site
+ * threads race for seeds on one atomic counter; here the draws are serialised
+ * in site order (site i draws from the seed after site i-1's last draw), the
+ * rule the HIP library implements.  Returns one past the last seed drawn. */
+ORC_API uint32_t orc_init_gradient_weighted(SadSite* sites, uint32_t n, uint32_t seed0, const float* target,
+                                            const float* mask, float thr, uint32_t attempts, float log_tau0,
+                                            float radius0, int width, int height)
+{
+    uint32_t seed = seed0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float px = 0.0f, py = 0.0f, g = 0.0f;
+        int pix = -1;
+        for (uint32_t a = 0; a < attempts; ++a) {
+            float x, y, gg;
+            int p;
+            if (sad_init_grad_eval(target, mask, width, height, seed++, thr, &x, &y, &p, &gg)) {
+                px = x; py = y; pix = p; g = gg;
+                break;
+            }
+        }
+        sites[i] = sad_init_grad_site(target, px, py, pix, g, log_tau0, radius0);
+    }
+    return seed;
+}
+
+/* one R2 seed: acceptance, position, pixel and gradient magnitude */
+ORC_API int orc_init_grad_eval(const float* target, const float* mask, int width, int height, uint32_t seed,
+                               float thr, float* out4)
+{
+    float x, y, g;
+    int p;
+    int ok = sad_init_grad_eval(target, mask, width, height, seed, thr, &x, &y, &p, &g);
+    out4[0] = x; out4[1] = y; out4[2] = (float)p; out4[3] = g;
+    return ok;
+}

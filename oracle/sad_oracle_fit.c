@@ -37,6 +37,8 @@ void orc_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const flo
 void orc_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float,
               float, float, uint32_t, uint32_t, int, int);
 void orc_densify_pairs(const SadSite*, const float*, const float*, SadPair*, uint32_t, float, float, uint32_t);
+uint32_t orc_init_gradient_weighted(SadSite*, uint32_t, uint32_t, const float*, const float*, float, uint32_t, float,
+                                    float, int, int);
 void orc_prune_pairs(const SadSite*, const float*, SadPair*, uint32_t, float, uint32_t);
 void orc_radix_sort(SadPair*, uint32_t, uint32_t);
 void orc_write_indices(const SadPair*, uint32_t*, uint32_t);
@@ -51,7 +53,8 @@ void orc_site_stats(const uint32_t*, const uint32_t*, const float*, const float*
 double orc_sse(const float*, const float*, const float*, int, int);
 
 /* Option block: the fields of TrainingOptions (sad_cuda.cu:528-576) that the
- * fit schedule reads.  init_mode 0 = sites given by the caller, 1 = per-pixel. */
+ * fit schedule reads.  init_mode 0 = sites given by the caller, 1 = per-pixel,
+ * 2 = gradient-weighted (n_sites made by orc_init_gradient_weighted). */
 typedef struct OrcFitConfig {
     int iterations;
     float target_bpp;
@@ -70,6 +73,7 @@ typedef struct OrcFitConfig {
     float lr_scale, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
     float beta1, beta2, eps;
     int log_freq;
+    float init_gradient_alpha, init_log_tau, init_radius;
 } OrcFitConfig;
 
 typedef struct OrcFitResult {
@@ -256,7 +260,15 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     int actual = n0;
     uint32_t n = (uint32_t)actual;
     int active_est = 0;
-    for (int i = 0; i < n0; ++i) if (sites[i].pos_x >= 0.0f) active_est++;
+    if (cfg->init_mode == 2) {
+        /* sad_cuda.cu:2078-2095: threshold 0.01 * alpha, 256 attempts; the
+         * reference counts its n default-constructed host sites as active */
+        orc_init_gradient_weighted(sites, n, 0u, target, mask, 0.01f * cfg->init_gradient_alpha, 256u,
+                                   cfg->init_log_tau, cfg->init_radius, width, height);
+        active_est = n0;
+    } else {
+        for (int i = 0; i < n0; ++i) if (sites[i].pos_x >= 0.0f) active_est++;
+    }
 
     orc_init_candidates(cA0, cA1, n, 0u, per_pixel, cw, ch);
     orc_jfa_seed(cA0, sites, n, cw, ch, ds);

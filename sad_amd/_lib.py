@@ -34,7 +34,8 @@ class FitOptions(C.Structure):
         ("cand_hilbert_window", U32), ("cand_hilbert_probes", U32),
         ("lr_scale", F32), ("lr_pos", F32), ("lr_tau", F32), ("lr_radius", F32), ("lr_color", F32),
         ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32),
-        ("log_freq", I32), ("profile_gradients", I32),
+        ("log_freq", I32), ("profile_gradients", I32), ("init_gradient_alpha", F32), ("init_log_tau", F32),
+        ("init_radius", F32),
     ]
 
 
@@ -46,7 +47,14 @@ class FitResult(C.Structure):
         ("n_logs", I32), ("n_densify", I32), ("n_prune", I32), ("n_cand_updates", I32),
         ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
         ("grad_bytes", C.c_double), ("grad_ms_trace", C.POINTER(F32)), ("grad_ms_trace_cap", I32),
+        ("final_render", C.POINTER(F32)),
     ]
+
+
+def fit_result_dict(res: "FitResult") -> dict:
+    """SadhFitResult fields as a dict, without its host-pointer fields."""
+    skip = ("grad_ms_trace", "grad_ms_trace_cap", "final_render")
+    return {k: getattr(res, k) for k, _ in FitResult._fields_ if k not in skip}
 
 
 class BatchJob(C.Structure):
@@ -106,6 +114,7 @@ SIGNATURES = {
     "sadh_prune_sites_by_index": [P, P, U32, P],
     "sadh_compute_psnr": [P, P, P, P, I32, I32, P],
     "sadh_count_active_sites": [P, P, U32, P],
+    "sadh_init_gradient_weighted": [P, U32, P, P, P, F32, U32, F32, F32, I32, I32, P],
     "sadh_fit_plan": [C.POINTER(FitOptions), I32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32),
                       C.POINTER(I32)],
     "sadh_fit": [C.POINTER(FitOptions), P, P, I32, I32, P, I32, P, I32, C.POINTER(FitResult)],

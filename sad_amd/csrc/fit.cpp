@@ -142,7 +142,9 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     if (sites_cap < cap) return fail(SADH_ERR_CAPACITY, "fit: sites_io holds " + std::to_string(sites_cap) +
                                                            " sites, the plan needs " + std::to_string(cap));
     const int np = width * height;
+    if (o.init_mode < 0 || o.init_mode > 2) return fail(SADH_ERR_INVALID, "fit: init_mode must be 0, 1 or 2");
     const bool per_pixel = o.init_mode == 1;
+    const bool grad_init = o.init_mode == 2;
     const int n0 = per_pixel ? np : o.n_sites;
     const int ds = std::max(1, o.cand_downscale);
     const int cw = std::max(1, (width + ds - 1) / ds), ch = std::max(1, (height + ds - 1) / ds);
@@ -157,7 +159,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     }
     std::vector<SadSite> host_sites(cap);
     for (int i = 0; i < cap; ++i) {
-        if (i < n0) host_sites[i] = sites_io[i];
+        if (i < n0 && !grad_init) host_sites[i] = sites_io[i];
         else { std::memset(&host_sites[i], 0, sizeof(SadSite)); host_sites[i].pos_x = host_sites[i].pos_y = -1.0f; }
     }
 
@@ -225,7 +227,16 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     int actual = n0;
     uint32_t n = (uint32_t)actual;
     int active_est = 0;
-    for (int i = 0; i < n0; ++i) active_est += host_sites[i].pos_x >= 0.0f ? 1 : 0;
+    if (grad_init) {
+        /* sad_cuda.cu:2078-2095; the host copy the reference counts holds n
+         * default sites at (0, 0), so its active estimate is n */
+        HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(uint32_t), S));
+        SAD_TRY(launch_init_gradient(d_sites, n, d_count, d_target, d_mask, 0.01f * o.init_gradient_alpha, 256u,
+                                     o.init_log_tau, o.init_radius, width, height, S));
+        active_est = n0;
+    } else {
+        for (int i = 0; i < n0; ++i) active_est += host_sites[i].pos_x >= 0.0f ? 1 : 0;
+    }
 
     SAD_TRY(sadh_init_candidates(cA0, cA1, n, 0u, per_pixel ? 1 : 0, cw, ch, (sadh_stream_t)S));
     SAD_TRY(sadh_jfa_seed(cA0, d_sites, n, cw, ch, ds, (sadh_stream_t)S));
@@ -393,6 +404,11 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
 
     HIP_TRY(hipMemcpy(host_sites.data(), d_sites, sizeof(SadSite) * cap, hipMemcpyDeviceToHost));
     std::memcpy(sites_io, host_sites.data(), sizeof(SadSite) * cap);
+    if (res && res->final_render) {
+        SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
+        HIP_TRY(hipMemcpyAsync(res->final_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
+        HIP_TRY(hipStreamSynchronize(S));
+    }
     if (res) {
         res->final_psnr = final_psnr;
         res->best_psnr = best;

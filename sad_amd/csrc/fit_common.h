@@ -34,6 +34,8 @@ int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float
                  int, int, hipStream_t);
 int launch_sse(const float*, const float*, const float*, float*, int, int, hipStream_t);
 int launch_count_active(const SadSite*, uint32_t*, uint32_t, hipStream_t);
+int launch_init_gradient(SadSite*, uint32_t, uint32_t*, const float*, const float*, float, uint32_t, float, float, int,
+                         int, hipStream_t);
 int launch_densify_pairs(const SadSite*, const float*, const float*, SadPair*, uint32_t, float, float, uint32_t,
                          hipStream_t);
 int launch_prune_pairs(const SadSite*, const float*, SadPair*, uint32_t, float, uint32_t, hipStream_t);

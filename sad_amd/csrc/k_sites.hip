@@ -488,6 +488,121 @@ __global__ __launch_bounds__(256) void k_count_active(const SadSite* __restrict_
     if ((threadIdx.x & 63u) == 0u && b) atomicAdd(count, (uint32_t)__popcll(b));
 }
 
+/* ----------------------------------------------- gradient-weighted init --
+ * initGradientWeightedKernel, densify_prune.cu:5-83.  There, every site thread
+ * draws seeds from one atomic counter until one is accepted or max_attempts
+ * are spent, so which site gets which seed depends on thread timing.  Here the
+ * draws are serialised in site order: site i draws from the seed after site
+ * i-1's last draw.  When no site runs out of attempts the set of sites is the
+ * same as the reference's (the first n accepted seeds; every seed the counter
+ * handed out was drawn by some site); the order is seed order.
+ *
+ * Serial walk: from q (one past the previous accepted seed), the next accepted
+ * seed a is preceded by (a - q) / M failed sites of M draws each, then taken by
+ * the next site; q = a + 1.  One workgroup evaluates IG_ROUND consecutive
+ * seeds per round (IG_PER per thread, a bit each), and two block scans place
+ * each thread's accepted seeds: an exclusive max-scan gives the end of the
+ * previous accepted seed, an exclusive sum-scan the site index.  Failed sites
+ * are all equal and are written up front.  The walk ends when n sites are
+ * placed, or when the open gap after q holds the remaining sites' failures. */
+#define IG_THREADS 1024
+#define IG_PER 32
+#define IG_ROUND (IG_THREADS * IG_PER)
+
+struct InitGradArgs {
+    SadSite* sites;
+    uint32_t* seed_counter;    /* in: first seed; out: one past the last seed drawn (null: from 0) */
+    const float* target;
+    const float* mask;         /* null: all ones */
+    float thr, log_tau0, radius0;
+    uint32_t n, attempts;
+    int w, h;
+};
+
+/* exclusive scan over the workgroup (sum, or max of non-negative values) and the block total */
+template <bool MAX>
+__device__ __forceinline__ uint32_t ig_scan(uint32_t v, uint32_t* lds, uint32_t* total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (uint32_t o = 1u; o < 64u; o <<= 1) {
+        uint32_t t = __shfl_up(x, o, 64);
+        if (lane >= o) x = MAX ? (t > x ? t : x) : x + t;
+    }
+    uint32_t xe = __shfl_up(x, 1u, 64);
+    if (lane == 0u) xe = 0u;
+    if (lane == 63u) lds[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0u, tot = 0u;
+    for (uint32_t k = 0u; k < IG_THREADS / 64u; ++k) {
+        uint32_t y = lds[k];
+        if (k < wid) pre = MAX ? (y > pre ? y : pre) : pre + y;
+        tot = MAX ? (y > tot ? y : tot) : tot + y;
+    }
+    __syncthreads();
+    *total = tot;
+    return MAX ? (xe > pre ? xe : pre) : pre + xe;
+}
+
+__global__ __launch_bounds__(IG_THREADS) void k_init_gradient(InitGradArgs A)
+{
+    __shared__ uint32_t lds[IG_THREADS / 64u];
+    const uint32_t tid = threadIdx.x;
+    const SadSite failed = sad_init_grad_site(A.target, 0.0f, 0.0f, -1, 0.0f, A.log_tau0, A.radius0);
+    for (uint32_t i = tid; i < A.n; i += IG_THREADS) A.sites[i] = failed;
+    if (A.attempts == 0u) return;       /* no draws: every site fails, the counter is unchanged */
+    __syncthreads();
+
+    const uint64_t M = A.attempts;
+    uint64_t p = A.seed_counter ? *A.seed_counter : 0u;   /* first seed of this round */
+    uint64_t q = p;                                       /* one past the last accepted seed */
+    uint32_t E = 0u;                                      /* sites placed */
+    for (;;) {
+        const uint64_t s0 = p + (uint64_t)tid * IG_PER;
+        uint32_t bits = 0u;
+        for (uint32_t j = 0u; j < IG_PER; ++j) {
+            float px, py, g;
+            int pix;
+            if (sad_init_grad_eval(A.target, A.mask, A.w, A.h, (uint32_t)(s0 + j), A.thr, &px, &py, &pix, &g))
+                bits |= 1u << j;
+        }
+        const uint32_t c = (uint32_t)__popc(bits);
+        const uint32_t first = tid * IG_PER + (c ? (uint32_t)__ffs(bits) - 1u : 0u);
+        const uint32_t end = c ? tid * IG_PER + 32u - (uint32_t)__clz(bits) : 0u;   /* one past the last */
+        uint32_t max_end, placed;
+        const uint32_t prev = ig_scan<true>(end, lds, &max_end);
+        const uint64_t prev_end = prev ? p + prev : q;
+        const uint32_t f = c ? (uint32_t)((p + first - prev_end) / M) : 0u;
+        const uint32_t base = ig_scan<false>(c + f, lds, &placed);
+        if (c) {
+            uint32_t i = E + base;
+            if (i < A.n && A.n - i <= f && A.seed_counter)   /* the last site is one of these failures */
+                *A.seed_counter = (uint32_t)(prev_end + (uint64_t)(A.n - i) * M);
+            i += f;
+            uint32_t b = bits;
+            while (b && i < A.n) {
+                const uint32_t j = (uint32_t)__ffs(b) - 1u;
+                b &= b - 1u;
+                const uint64_t s = s0 + j;
+                float px, py, g;
+                int pix;
+                sad_init_grad_eval(A.target, A.mask, A.w, A.h, (uint32_t)s, A.thr, &px, &py, &pix, &g);
+                A.sites[i] = sad_init_grad_site(A.target, px, py, pix, g, A.log_tau0, A.radius0);
+                if (i == A.n - 1u && A.seed_counter) *A.seed_counter = (uint32_t)(s + 1u);
+                ++i;
+            }
+        }
+        E += placed;
+        if (max_end) q = p + max_end;
+        p += IG_ROUND;
+        if (E >= A.n) return;
+        if ((p - q) / M >= (uint64_t)(A.n - E)) {     /* the open gap fails every remaining site */
+            if (tid == 0u && A.seed_counter) *A.seed_counter = (uint32_t)(q + (uint64_t)(A.n - E) * M);
+            return;
+        }
+    }
+}
+
 /* ------------------------------------------------------------ launchers -- */
 
 static void fill_adam(AdamArgs& A, SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos, float lr_tau,
@@ -706,6 +821,19 @@ int launch_hilbert_order(const SadPair* sorted, uint32_t* order, uint32_t* pos, 
     return launch_status("hilbert_order");
 }
 
+int launch_init_gradient(SadSite* sites, uint32_t n, uint32_t* seed_counter, const float* target, const float* mask,
+                         float thr, uint32_t attempts, float log_tau0, float radius0, int width, int height,
+                         hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    InitGradArgs A;
+    A.sites = sites; A.seed_counter = seed_counter; A.target = target; A.mask = mask;
+    A.thr = thr; A.log_tau0 = log_tau0; A.radius0 = radius0;
+    A.n = n; A.attempts = attempts; A.w = width; A.h = height;
+    hipLaunchKernelGGL(k_init_gradient, dim3(1), dim3(IG_THREADS), 0, stream, A);
+    return launch_status("init_gradient_weighted");
+}
+
 }  // namespace sadh
 
 using namespace sadh;
@@ -863,6 +991,16 @@ int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site
 {
     SADH_REQUIRE(sites && count, "count_active_sites: null pointer");
     return launch_count_active(sites, count, site_count, to_hip(stream));
+}
+
+int sadh_init_gradient_weighted(SadSite* sites, uint32_t num_sites, uint32_t* seed_counter, const float* target,
+                                const float* mask, float grad_threshold, uint32_t max_attempts, float init_log_tau,
+                                float init_radius, int width, int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && target && width > 0 && height > 0, "init_gradient_weighted: bad arguments");
+    SADH_REQUIRE(num_sites <= 0x7FFFFFFFu, "init_gradient_weighted: too many sites");
+    return launch_init_gradient(sites, num_sites, seed_counter, target, mask, grad_threshold, max_attempts,
+                                init_log_tau, init_radius, width, height, to_hip(stream));
 }
 
 }  // extern "C"

@@ -443,4 +443,67 @@ SAD_HD int sad_hilbert_bits(int width, int height)
     return bits > 1 ? bits : 1;
 }
 
+/* ------------------------------------------- gradient-weighted init ----
+ * Seed s of the R2 low-discrepancy sequence (g = plastic number) lands at
+ * uv = frac(0.5 + s * (1/g, 1/g^2)), pixel position uv * (W-1, H-1).  The
+ * seed is accepted when the mask there is positive and the mask-weighted
+ * Sobel magnitude of the luma (0.299, 0.587, 0.114) exceeds the threshold
+ * (initGradientWeightedKernel, densify_prune.cu:5-83).  The multiply-adds are
+ * written as the fused operations a contracting compiler emits for them. */
+#define SAD_R2_G 1.32471795724474602596f
+
+SAD_HD float sad_init_luma(const float* target, int w, int h, int x, int y)
+{
+    const float* p = target + 3 * ((size_t)sad_clampi(y, 0, h - 1) * (size_t)w + (size_t)sad_clampi(x, 0, w - 1));
+    return fmaf(0.114f, p[2], fmaf(0.587f, p[1], 0.299f * p[0]));
+}
+
+/* -> 1 when seed s is accepted; position and gradient magnitude out */
+SAD_HD int sad_init_grad_eval(const float* target, const float* mask, int w, int h, uint32_t seed, float thr,
+                              float* px, float* py, int* pix, float* grad)
+{
+    const float a1 = 1.0f / SAD_R2_G;
+    const float a2 = 1.0f / (SAD_R2_G * SAD_R2_G);
+    const float fs = (float)seed;
+    float ux = fmaf(fs, a1, 0.5f), uy = fmaf(fs, a2, 0.5f);
+    ux -= floorf(ux);                 /* fmodf(u, 1) for u >= 0, exactly */
+    uy -= floorf(uy);
+    const float x = ux * (float)(w - 1), y = uy * (float)(h - 1);
+    const int ix = (int)x, iy = (int)y;
+    *px = x;
+    *py = y;
+    *pix = iy * w + ix;
+    *grad = 0.0f;
+    const float m = mask ? mask[(size_t)iy * (size_t)w + (size_t)ix] : 1.0f;
+    if (m <= 0.0f) return 0;
+    const float tl = sad_init_luma(target, w, h, ix - 1, iy - 1), t = sad_init_luma(target, w, h, ix, iy - 1);
+    const float tr = sad_init_luma(target, w, h, ix + 1, iy - 1), l = sad_init_luma(target, w, h, ix - 1, iy);
+    const float r = sad_init_luma(target, w, h, ix + 1, iy), bl = sad_init_luma(target, w, h, ix - 1, iy + 1);
+    const float b = sad_init_luma(target, w, h, ix, iy + 1), br = sad_init_luma(target, w, h, ix + 1, iy + 1);
+    const float gx = (((((-tl + tr) - 2.0f * l) + 2.0f * r) - bl) + br) / 4.0f;
+    const float gy = (((((-tl - 2.0f * t) - tr) + bl) + 2.0f * b) + br) / 4.0f;
+    const float g = sqrtf(fmaf(gy, gy, gx * gx)) * m;
+    *grad = g;
+    return g > thr;
+}
+
+/* Site made from an accepted seed (or, pix < 0, from none): log_tau grows with
+ * the fourth root of the gradient, taken as two correctly rounded square roots. */
+SAD_HD SadSite sad_init_grad_site(const float* target, float px, float py, int pix, float grad, float log_tau0,
+                                  float radius0)
+{
+    SadSite s;
+    s.pos_x = pix >= 0 ? px : -1.0f;
+    s.pos_y = pix >= 0 ? py : -1.0f;
+    s.log_tau = log_tau0 + sqrtf(sqrtf(grad)) * 2.0f;
+    s.radius = radius0;
+    s.col_r = pix >= 0 ? target[3 * (size_t)pix + 0] : 0.5f;
+    s.col_g = pix >= 0 ? target[3 * (size_t)pix + 1] : 0.5f;
+    s.col_b = pix >= 0 ? target[3 * (size_t)pix + 2] : 0.5f;
+    s.dir_x = 1.0f;
+    s.dir_y = 0.0f;
+    s.log_aniso = 0.0f;
+    return s;
+}
+
 #endif /* SAD_MATH_H */

@@ -13,13 +13,14 @@ fails in the dispatcher.
 Differences from the reference wrappers:
   * work is issued on the current stream rather than the device's default
     stream, so the ops can be captured in HIP graphs and run on side streams;
-  * `init_gradient_weighted` raises NotImplementedError (gradient-weighted
-    init is not part of this library);
+  * `init_gradient_weighted` hands out the R2 seeds in site order, where the
+    reference's threads race for them (same sites, deterministic order);
   * `compute_gradients_tiled` with ssim_weight > 0 fails with the library's
     "unsupported" status (the SSIM term is not implemented).
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -521,8 +522,24 @@ def write_hilbert_order(pairs, order, pos, site_count):
 
 
 def init_gradient_weighted(sites, target, mask, site_count, init_log_tau, init_radius, gradient_alpha):
-    """initGradientWeighted (densify_prune.cu:5-83) is not part of this library."""
-    raise NotImplementedError("sad::init_gradient_weighted is not implemented by the HIP backend")
+    """initGradientWeighted, sad_bindings.cu:1399-1437: threshold 0.01 * alpha,
+    256 attempts, seeds from 0.  Sites receive their seeds in site order (the
+    reference's threads race for them), so the result is deterministic."""
+    _sites(sites)
+    _image(target, "target")
+    _cuda(mask, "mask")
+    _f32(mask, "mask")
+    _check(mask.dim() == 2, "mask must be [H,W]")
+    _check(mask.size(0) == target.size(0) and mask.size(1) == target.size(1), "mask size must match target")
+    sites, target, mask = sites.contiguous(), target.contiguous(), mask.contiguous()
+    if site_count <= 0:
+        site_count = sites.size(0)
+    _check(site_count <= sites.size(0), "site_count exceeds sites rows")
+    counter = torch.zeros(1, dtype=torch.int32, device=sites.device)
+    _run("sadh_init_gradient_weighted", sites, _p(sites), site_count, _p(counter), _p(target), _p(mask),
+         float(np.float32(0.01) * np.float32(gradient_alpha)), 256, float(init_log_tau), float(init_radius),
+         target.size(1), target.size(0))
+    return sites
 
 
 _IMPLS = {

@@ -108,7 +108,7 @@ class Server:
         if r["kind"] == "render":
             return dict(image=r["image"], **times)
         res = r["res"]
-        out = {k: getattr(res, k) for k, _ in _lib.FitResult._fields_ if not k.startswith("grad_ms_trace")}
+        out = _lib.fit_result_dict(res)
         out["sites"] = r["sites"]
         out["log_psnr"] = r["logs"][: min(res.n_logs, len(r["logs"]))].copy()
         out.update(times)

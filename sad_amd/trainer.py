@@ -17,23 +17,26 @@ def fit_plan(opts: dict, width: int, height: int) -> dict:
     return dict(densify_percentile=d.value, prune_percentile=p.value, capacity=cap.value, pair_count=pairs.value)
 
 
-def fit(target: np.ndarray, sites: np.ndarray, opts: dict | None = None, mask: np.ndarray | None = None,
-        **over) -> dict:
+def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, mask: np.ndarray | None = None,
+        want_render: bool = False, **over) -> dict:
     """Run the fit on the current HIP device.
 
-    target float32 [H][W][3]; sites float32 [N][10] initial sites.  Returns a
-    dict with the final sites [capacity][10], logged PSNRs and the SadhFitResult
-    fields."""
+    target float32 [H][W][3]; sites float32 [N][10] initial sites (None with
+    init_mode 2, where the fit makes n_sites gradient-weighted sites).  Returns
+    a dict with the final sites [capacity][10], logged PSNRs and the
+    SadhFitResult fields."""
     o = dict(opts or fit_options())
     o.update(over)
     h, w, _ = target.shape
     if o["init_mode"] == 0:
         o["n_sites"] = int(sites.shape[0])
     plan = fit_plan(o, w, h)
-    cap = max(plan["capacity"], sites.shape[0])
+    n_in = 0 if sites is None else sites.shape[0]
+    cap = max(plan["capacity"], n_in)
     buf = np.zeros((cap, 10), np.float32)
     buf[:, 0:2] = -1.0
-    buf[: sites.shape[0]] = sites
+    if n_in:
+        buf[:n_in] = sites
     tgt = np.ascontiguousarray(target, np.float32)
     msk = None if mask is None else np.ascontiguousarray(mask, np.float32)
     logs = np.zeros(64, np.float32)
@@ -42,11 +45,15 @@ def fit(target: np.ndarray, sites: np.ndarray, opts: dict | None = None, mask: n
     if trace is not None:
         res.grad_ms_trace = trace.ctypes.data_as(C.POINTER(C.c_float))
         res.grad_ms_trace_cap = trace.size
+    render = np.zeros((h, w, 3), np.float32) if want_render else None
+    if render is not None:
+        res.final_render = render.ctypes.data_as(C.POINTER(C.c_float))
     s = to_struct(o)
     _lib.call("sadh_fit", C.byref(s), _lib.ptr(tgt), _lib.ptr(msk), w, h, _lib.ptr(buf), cap, _lib.ptr(logs),
               len(logs), C.byref(res))
-    out = {k: getattr(res, k) for k, _ in _lib.FitResult._fields_ if not k.startswith("grad_ms_trace")}
+    out = _lib.fit_result_dict(res)
     out["grad_ms_trace"] = trace
+    out["render"] = render
     out["sites"] = buf
     out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
     out["plan"] = plan
@@ -95,7 +102,7 @@ def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list 
     s = to_struct(o)
     _lib.call("sadh_fit_batch", C.byref(s), jobs, n, C.byref(br))
     for out, res in zip(outs, results):
-        out.update({k: getattr(res, k) for k, _ in _lib.FitResult._fields_ if not k.startswith("grad_ms_trace")})
+        out.update(_lib.fit_result_dict(res))
         logs = out.pop("_logs")
         out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
     ret = {k: getattr(br, k) for k, _ in _lib.BatchResult._fields_}
