@@ -31,7 +31,7 @@ typedef struct ihipStream_t* sadh_stream_t;
 
 #define SADH_OK 0
 #define SADH_ERR_INVALID (-1)      /* bad size, null pointer or out-of-range scalar */
-#define SADH_ERR_UNSUPPORTED (-2)  /* option this build does not implement (e.g. SSIM loss) */
+#define SADH_ERR_UNSUPPORTED (-2)  /* option this build does not implement */
 #define SADH_ERR_CAPACITY (-3)     /* request exceeds a buffer capacity */
 #define SADH_ERR_NO_DEVICE (-4)    /* no usable gfx950 device */
 
@@ -102,8 +102,11 @@ int sadh_hilbert_order(const SadPair* sorted_pairs, uint32_t* order, uint32_t* p
 
 /* --------------------------------------------------------- gradients ----- */
 
-/* launchComputeGradientsTiled, gradients.cu:491-531.  `rendered` is read only by
- * the SSIM term; ssim_weight > 0 returns SADH_ERR_UNSUPPORTED in this build. */
+/* launchComputeGradientsTiled, gradients.cu:491-531.  `rendered` (the render of
+ * the current sites) is read only by the SSIM term, ssim_weight > 0: each
+ * masked pixel adds -ssim_weight * dSSIM/dx_k * (0.299, 0.587, 0.114) to
+ * dL/dpred, the derivative of its own 3x3 window's SSIM through its own luma
+ * (gradients.cu:236-266). */
 int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1,
                                  const float* target, const float* rendered, const float* mask,
                                  float* grad_pos_x, float* grad_pos_y, float* grad_log_tau,
@@ -229,6 +232,11 @@ int sadh_prune_sites_by_index(SadSite* sites, const uint32_t* indices, uint32_t 
 int sadh_compute_psnr(const float* rendered, const float* target, const float* mask,
                       float* mse_accum, int width, int height, sadh_stream_t stream);
 
+/* launchComputeSSIM, utils.cu:279-297: *ssim_accum = sum over masked pixels of
+ * the SSIM of the pixel's 3x3 luma window (clamped at the borders). */
+int sadh_compute_ssim(const float* rendered, const float* target, const float* mask,
+                      float* ssim_accum, int width, int height, sadh_stream_t stream);
+
 /* launchCountActiveSites, utils.cu:637-651 */
 int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count,
                             sadh_stream_t stream);
@@ -274,6 +282,9 @@ typedef struct SadhFitOptions {
     int profile_gradients;            /* 1: time every gradient launch with HIP events */
     float init_gradient_alpha;        /* init_mode 2: threshold = 0.01 * alpha */
     float init_log_tau, init_radius;  /* init_mode 2: initial log_tau and radius */
+    float ssim_weight;                /* > 0: loss L2 - w * SSIM, with a render before each gradient
+                                         (--ssim-weight, sad_cuda.cu:2307-2330); sadh_fit only */
+    int ssim_metric;                  /* 1: log the mean SSIM beside the PSNR (--ssim, sad_cuda.cu:2432-2447) */
 } SadhFitOptions;
 
 typedef struct SadhFitResult {
@@ -290,6 +301,8 @@ typedef struct SadhFitResult {
     int grad_ms_trace_cap;            /* in: entries of grad_ms_trace */
     float* final_render;              /* in: optional host [H][W][3]; out: render of the final sites with the
                                          final candidates (sad_cuda.cu:2482-2496); sadh_fit only */
+    float final_ssim, best_ssim;      /* ssim_metric: mean SSIM at the last log, and the best */
+    float* log_ssim;                  /* in: optional host array of log_cap entries; out: the logged SSIMs */
 } SadhFitResult;
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does

@@ -30,7 +30,8 @@ class OrcFitConfig(C.Structure):
         ("cand_hilbert_window", U32), ("cand_hilbert_probes", U32),
         ("lr_scale", F32), ("lr_pos", F32), ("lr_tau", F32), ("lr_radius", F32), ("lr_color", F32),
         ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32), ("log_freq", I32),
-        ("init_gradient_alpha", F32), ("init_log_tau", F32), ("init_radius", F32),
+        ("init_gradient_alpha", F32), ("init_log_tau", F32), ("init_radius", F32), ("ssim_weight", F32),
+        ("ssim_metric", I32),
     ]
 
 
@@ -38,7 +39,7 @@ class OrcFitResult(C.Structure):
     _fields_ = [
         ("final_psnr", F32), ("best_psnr", F32), ("actual_sites", I32), ("active_estimate", I32),
         ("densify_percentile", F32), ("prune_percentile", F32), ("capacity", I32), ("n_logs", I32),
-        ("n_densify", I32), ("n_prune", I32), ("n_cand_updates", I32),
+        ("n_densify", I32), ("n_prune", I32), ("n_cand_updates", I32), ("final_ssim", F32), ("best_ssim", F32),
     ]
 
 
@@ -53,6 +54,7 @@ def lib() -> C.CDLL:
             build()
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_sse.restype = C.c_double
+        _lib.orc_ssim_sum.restype = C.c_double
         _lib.orc_hilbert_index.restype = U32
         _lib.orc_jump_step.restype = U32
         _lib.orc_get_threads.restype = I32
@@ -166,14 +168,21 @@ def update_candidates(cand0, cand1, packed, step, iss, width, height, step_high=
 
 
 def gradients(cand0, cand1, target, mask, sites, iss, width, height, compute_removal=False, cand_w=None,
-              cand_h=None):
-    """Returns (grad [N,10] float64, abs-sum [N,10], removal [N], removal abs-sum [N])."""
+              cand_h=None, rendered=None, ssim_weight=0.0):
+    """Returns (grad [N,10] float64, abs-sum [N,10], removal [N], removal abs-sum [N]).
+    With ssim_weight > 0 the loss gains -ssim_weight * SSIM, read from `rendered`."""
     c0, c1, t, m, s = _u32(cand0), _u32(cand1), _f32(target), _f32(mask), _f32(sites)
     n = s.shape[0]
     acc = np.zeros((n, 10)); ab = np.zeros((n, 10)); rd = np.zeros(n); ra = np.zeros(n)
-    lib().orc_gradients(_p(c0), _p(c1), _p(t), _p(m), _p(s), F32(iss), U32(n), I32(int(compute_removal)),
-                        I32(width), I32(height), I32(cand_w or width), I32(cand_h or height), _p(acc), _p(ab),
-                        _p(rd), _p(ra))
+    if ssim_weight > 0.0:
+        r = _f32(rendered)
+        lib().orc_gradients_ssim(_p(c0), _p(c1), _p(t), _p(r), _p(m), _p(s), F32(iss), U32(n),
+                                 I32(int(compute_removal)), F32(ssim_weight), I32(width), I32(height),
+                                 I32(cand_w or width), I32(cand_h or height), _p(acc), _p(ab), _p(rd), _p(ra))
+    else:
+        lib().orc_gradients(_p(c0), _p(c1), _p(t), _p(m), _p(s), F32(iss), U32(n), I32(int(compute_removal)),
+                            I32(width), I32(height), I32(cand_w or width), I32(cand_h or height), _p(acc), _p(ab),
+                            _p(rd), _p(ra))
     return acc, ab, rd, ra
 
 
@@ -283,14 +292,16 @@ def fit(target, sites, opts: dict, mask=None) -> dict:
     t = _f32(target)
     m = None if mask is None else _f32(mask)
     logs = np.zeros(64, np.float32)
+    slogs = np.zeros(64, np.float32)
     res = OrcFitResult()
     rc = lib().orc_fit(C.byref(_cfg(o)), _p(t), _p(m), I32(w), I32(h), _p(buf), I32(cap), _p(logs), I32(64),
-                       C.byref(res))
+                       C.byref(res), _p(slogs))
     if rc != 0:
         raise RuntimeError(f"orc_fit failed: {rc}")
     out = {k: getattr(res, k) for k, _ in OrcFitResult._fields_}
     out["sites"] = buf
     out["log_psnr"] = logs[: min(res.n_logs, 64)].copy()
+    out["log_ssim"] = slogs[: min(res.n_logs, 64)].copy()
     out["plan"] = plan
     return out
 
@@ -314,3 +325,23 @@ def init_grad_eval(target, seed, mask=None, threshold=0.01):
     o = np.zeros(4, np.float32)
     ok = lib().orc_init_grad_eval(_p(t), _p(m), I32(w), I32(h), U32(seed), F32(threshold), _p(o))
     return bool(ok), float(o[0]), float(o[1]), int(o[2]), float(o[3])
+
+
+def ssim_sum(rendered, target, mask=None):
+    """-> (sum over masked pixels of the window SSIM, per-pixel SSIM [H][W])"""
+    r, t = _f32(rendered), _f32(target)
+    h, w, _ = r.shape
+    m = np.ones((h, w), np.float32) if mask is None else _f32(mask)
+    per = np.zeros((h, w), np.float32)
+    s = lib().orc_ssim_sum(_p(r), _p(t), _p(m), I32(w), I32(h), _p(per))
+    return float(s), per
+
+
+def ssim_dcentre(rendered, target, mask=None):
+    """dSSIM(window of k)/dx_k per pixel [H][W]"""
+    r, t = _f32(rendered), _f32(target)
+    h, w, _ = r.shape
+    m = np.ones((h, w), np.float32) if mask is None else _f32(mask)
+    out = np.zeros((h, w), np.float32)
+    lib().orc_ssim_dcentre(_p(r), _p(t), _p(m), I32(w), I32(h), _p(out))
+    return out

@@ -310,8 +310,10 @@ ORC_API void orc_update_candidates(const uint32_t* in0, const uint32_t* in1,
 }
 
 /* -------------------------------------------------------------- gradients --
- * computeGradientsTiledKernel, gradients.cu:103-486 (L2 loss; the SSIM branch
- * is not restated: ssim_weight = 0 on this path).  The per-candidate analytic
+ * computeGradientsTiledKernel, gradients.cu:103-486: L2 loss, plus
+ * -ssim_weight * SSIM when ssim_weight > 0 (gradients.cu:236-266: each pixel's
+ * own 3x3 window SSIM differentiated through its own rendered luma, read from
+ * `rendered`).  The per-candidate analytic
  * derivative of gradients.cu:352-420 is sad_grad_terms (sad_math.h), which is
  * pinned separately against central finite differences of the render; the
  * tile hash is an accumulation device only, so the oracle sums per site directly.
@@ -341,7 +343,8 @@ static void orc_gradients_impl(int mode, const uint32_t* cand0, const uint32_t* 
                                const float* target, const float* mask, const float* grad_out,
                                const SadSite* sites, float iss, uint32_t n,
                                int compute_removal, int width, int height, int cand_w, int cand_h,
-                               double* acc, double* abs_acc, double* rdelta, double* rabs)
+                               double* acc, double* abs_acc, double* rdelta, double* rabs,
+                               const float* rendered, float ssim_weight)
 {
     int T = omp_get_max_threads();
     size_t nn = (size_t)n;
@@ -385,6 +388,13 @@ static void orc_gradients_impl(int mode, const uint32_t* cand0, const uint32_t* 
                     g[0] = 2.0f * (pred[0] - tgt[0]);
                     g[1] = 2.0f * (pred[1] - tgt[1]);
                     g[2] = 2.0f * (pred[2] - tgt[2]);
+                    if (ssim_weight > 0.0f) {
+                        SadSsimStats st = sad_ssim_window(rendered, target, width, height, x, y);
+                        float f = -ssim_weight * sad_ssim_dcentre(st, sad_luma3(rendered + 3 * p), sad_luma3(tgt));
+                        g[0] += f * 0.299f;
+                        g[1] += f * 0.587f;
+                        g[2] += f * 0.114f;
+                    }
                 } else {
                     g[0] = grad_out[3 * p]; g[1] = grad_out[3 * p + 1]; g[2] = grad_out[3 * p + 2];
                 }
@@ -435,7 +445,17 @@ ORC_API void orc_gradients(const uint32_t* cand0, const uint32_t* cand1, const f
                            double* acc, double* abs_acc, double* rdelta, double* rabs)
 {
     orc_gradients_impl(0, cand0, cand1, target, mask, NULL, sites, iss, n, compute_removal,
-                       width, height, cand_w, cand_h, acc, abs_acc, rdelta, rabs);
+                       width, height, cand_w, cand_h, acc, abs_acc, rdelta, rabs, NULL, 0.0f);
+}
+
+/* the tiled gradient with the SSIM loss term (rendered = the current render) */
+ORC_API void orc_gradients_ssim(const uint32_t* cand0, const uint32_t* cand1, const float* target,
+                                const float* rendered, const float* mask, const SadSite* sites, float iss,
+                                uint32_t n, int compute_removal, float ssim_weight, int width, int height,
+                                int cand_w, int cand_h, double* acc, double* abs_acc, double* rdelta, double* rabs)
+{
+    orc_gradients_impl(0, cand0, cand1, target, mask, NULL, sites, iss, n, compute_removal,
+                       width, height, cand_w, cand_h, acc, abs_acc, rdelta, rabs, rendered, ssim_weight);
 }
 
 /* render_sad_backward_kernel, sad_bindings.cu:100-207: grad_sites [N,10] AoS */
@@ -444,7 +464,7 @@ ORC_API void orc_render_backward(const uint32_t* cand0, const uint32_t* cand1, c
                                  int cand_w, int cand_h, double* acc, double* abs_acc)
 {
     orc_gradients_impl(1, cand0, cand1, NULL, NULL, grad_out, sites, iss, n, 0,
-                       width, height, cand_w, cand_h, acc, abs_acc, NULL, NULL);
+                       width, height, cand_w, cand_h, acc, abs_acc, NULL, NULL, NULL, 0.0f);
 }
 
 /* tau_diffuse_kernel, tau_diffusion.cu:27-89 */
@@ -785,6 +805,40 @@ ORC_API double orc_sse(const float* rendered, const float* target, const float* 
         s += (double)(d0 * d0 + d1 * d1 + d2 * d2);
     }
     return s;
+}
+
+/* computeSSIMKernel, utils.cu:29-87: sum over masked pixels of the window SSIM
+ * (double accumulation); per-pixel values into per_px when non-null */
+ORC_API double orc_ssim_sum(const float* rendered, const float* target, const float* mask, int width, int height,
+                            float* per_px)
+{
+    double s = 0.0;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            size_t p = (size_t)y * width + x;
+            float v = 0.0f;
+            if (mask[p] > 0.0f) {
+                float A, B, C, D;
+                v = sad_ssim_terms(sad_ssim_window(rendered, target, width, height, x, y), &A, &B, &C, &D);
+                s += (double)v;
+            }
+            if (per_px) per_px[p] = v;
+        }
+    return s;
+}
+
+/* the SSIM loss factor -w dSSIM/dx_k of every pixel (0 outside the mask) */
+ORC_API void orc_ssim_dcentre(const float* rendered, const float* target, const float* mask, int width, int height,
+                              float* out)
+{
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            size_t p = (size_t)y * width + x;
+            out[p] = 0.0f;
+            if (mask[p] > 0.0f)
+                out[p] = sad_ssim_dcentre(sad_ssim_window(rendered, target, width, height, x, y),
+                                          sad_luma3(rendered + 3 * p), sad_luma3(target + 3 * p));
+        }
 }
 
 /* initGradientWeightedKernel, densify_prune.cu:5-83.  The reference's site

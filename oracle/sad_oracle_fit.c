@@ -51,6 +51,9 @@ void orc_hilbert_order(const SadPair*, uint32_t*, uint32_t*, uint32_t);
 void orc_site_stats(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float,
                     uint32_t, int, int, int, int, double*);
 double orc_sse(const float*, const float*, const float*, int, int);
+double orc_ssim_sum(const float*, const float*, const float*, int, int, float*);
+void orc_gradients_ssim(const uint32_t*, const uint32_t*, const float*, const float*, const float*, const SadSite*,
+                        float, uint32_t, int, float, int, int, int, int, double*, double*, double*, double*);
 
 /* Option block: the fields of TrainingOptions (sad_cuda.cu:528-576) that the
  * fit schedule reads.  init_mode 0 = sites given by the caller, 1 = per-pixel,
@@ -74,6 +77,8 @@ typedef struct OrcFitConfig {
     float beta1, beta2, eps;
     int log_freq;
     float init_gradient_alpha, init_log_tau, init_radius;
+    float ssim_weight;             /* > 0: loss L2 - w SSIM, render before each gradient */
+    int ssim_metric;               /* 1: log the mean SSIM */
 } OrcFitConfig;
 
 typedef struct OrcFitResult {
@@ -83,6 +88,7 @@ typedef struct OrcFitResult {
     int capacity;
     int n_logs;
     int n_densify, n_prune, n_cand_updates;
+    float final_ssim, best_ssim;
 } OrcFitResult;
 
 static int imax(int a, int b) { return a > b ? a : b; }
@@ -205,7 +211,7 @@ ORC_API void orc_fit_plan(const OrcFitConfig* cfg, int width, int height, float*
  * log_psnr (nullable, log_cap entries) receives each logged PSNR. */
 ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* mask_in,
                     int width, int height, SadSite* sites_io, int sites_cap,
-                    float* log_psnr, int log_cap, OrcFitResult* res)
+                    float* log_psnr, int log_cap, OrcFitResult* res, float* log_ssim)
 {
     int np = width * height;
     float d_pct, p_pct;
@@ -288,7 +294,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     int needs_pairs = pairs_count > 0;
     int needs_prune = p_pct > 0.0f;
     uint32_t jump_index = 0;
-    float best = 0.0f, final_psnr = 0.0f;
+    float best = 0.0f, final_psnr = 0.0f, best_ssim = 0.0f, final_ssim = 0.0f;
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
 
     for (int iter = 0; iter < cfg->iterations; ++iter) {
@@ -353,8 +359,15 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
 
         memset(acc, 0, sizeof(double) * 10 * n);
         if (prune) memset(racc, 0, sizeof(double) * n);
-        orc_gradients(gA0, gA1, target, mask, sites, iss, n, prune, width, height, cw, ch,
-                      acc, NULL, prune ? racc : NULL, NULL);
+        if (cfg->ssim_weight > 0.0f) {
+            /* sad_cuda.cu:2307-2330: render the current state, then the gradient reads it */
+            orc_render(gA0, gA1, render, sites, iss, n, width, height, cw, ch);
+            orc_gradients_ssim(gA0, gA1, target, render, mask, sites, iss, n, prune, cfg->ssim_weight, width,
+                               height, cw, ch, acc, NULL, prune ? racc : NULL, NULL);
+        } else {
+            orc_gradients(gA0, gA1, target, mask, sites, iss, n, prune, width, height, cw, ch,
+                          acc, NULL, prune ? racc : NULL, NULL);
+        }
         for (uint32_t i = 0; i < n; ++i) {
             for (int j = 0; j < 10; ++j) grads[j][i] += (float)acc[10 * (size_t)i + j];
             if (prune) removal[i] += (float)racc[i];
@@ -393,6 +406,12 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
             float mse = sse / (fmaxf(1.0f, mask_sum) * 3.0f);
             float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / sqrtf(mse)) : 100.0f;
             if (log_psnr && n_logs < log_cap) log_psnr[n_logs] = psnr;
+            if (cfg->ssim_metric) {
+                float ssim = (float)orc_ssim_sum(render, target, mask, width, height, NULL) / fmaxf(1.0f, mask_sum);
+                if (log_ssim && n_logs < log_cap) log_ssim[n_logs] = ssim;
+                final_ssim = ssim;
+                if (ssim > best_ssim) best_ssim = ssim;
+            }
             n_logs++;
             final_psnr = psnr;
             if (psnr > best) best = psnr;
@@ -402,6 +421,8 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     if (res) {
         res->final_psnr = final_psnr;
         res->best_psnr = best;
+        res->final_ssim = final_ssim;
+        res->best_ssim = best_ssim;
         res->actual_sites = actual;
         res->active_estimate = active_est;
         res->densify_percentile = d_pct;

@@ -8,8 +8,7 @@ Defaults come from training_config.json in the working directory, else the
 file named by $CONFIG_PATH, else the built-in copy of the reference's file.
 A fit writes <out-dir>/<stem>.png (the final render) and
 <out-dir>/<stem>_sites.txt (the active sites); a render writes --out, else
-<sites path without extension>_render.png.  Flags that select the SSIM metric
-or loss are rejected: this build has no SSIM kernel.
+<sites path without extension>_render.png.
 """
 from __future__ import annotations
 
@@ -56,6 +55,7 @@ Training options:
   --prune-during-densify  --prune-start/-end/-freq N  --prune-percentile F
   --cand-freq N  --cand-passes N  --cand-downscale N  --cand-radius-scale F  --cand-radius-probes N
   --cand-inject N  --cand-hilbert-window N  --cand-hilbert-probes N  --max-sites N  --lr F  --log-freq N
+  --ssim (log SSIM)  --ssim-weight F (loss L2 - F * SSIM)
 Render options:
   --out <path>  --width N  --height N  --render-cand-passes N  --render-no-jfa  --render-jfa-rounds N
   --render-cand-downscale N  --render-hilbert-window N  --render-hilbert-probes N  --render-cand-inject N
@@ -78,8 +78,8 @@ def parse_args(argv: list, cfg: dict) -> dict:
     o["init_mode"] = 2
     if cfg["DEFAULT_TARGET_BPP"] <= 0:
         o["target_bpp"] = 0.0
-    run = dict(mode="train", opts=o, target=None, mask=None, init_sites=None, out_dir="results", ssim=False,
-               ssim_weight=0.0, max_dim=int(cfg["MAX_DIM"]),
+    run = dict(mode="train", opts=o, target=None, mask=None, init_sites=None, out_dir="results",
+               max_dim=int(cfg["MAX_DIM"]),
                render=dict(sites=None, out=None, target=None, width=0, height=0, cand_passes=16, use_jfa=True,
                            jfa_rounds=1, cand_downscale=int(cfg["CAND_DOWNSCALE"]),
                            radius_scale=float(cfg["CAND_RADIUS_SCALE"]),
@@ -128,9 +128,9 @@ def parse_args(argv: list, cfg: dict) -> dict:
                 elif a == "--prune-during-densify":
                     o["prune_during_densify"] = 1
                 elif a == "--ssim":
-                    run["ssim"] = True
+                    o["ssim_metric"] = 1
                 elif a == "--ssim-weight":
-                    run["ssim_weight"] = float(value(a))
+                    o["ssim_weight"] = float(value(a))
                 elif a == "--log-freq":
                     o["log_freq"] = max(1, int(value(a)))
                 elif a == "--out-dir":
@@ -175,8 +175,6 @@ def train(run: dict, out=None) -> dict:
     from . import trainer
     out = out or sys.stdout
     o = dict(run["opts"])
-    if run["ssim"] or run["ssim_weight"] > 0.0:
-        raise UsageError("--ssim / --ssim-weight: this build has no SSIM kernel")
     t_start = time.perf_counter()
     target = sites_io.load_image(run["target"], run["max_dim"])
     h, w, _ = target.shape
@@ -203,9 +201,10 @@ def train(run: dict, out=None) -> dict:
     res = trainer.fit(target, init, o, mask=mask, want_render=True)
     logs = list(res["log_psnr"])
     its = [k for k in range(o["iterations"]) if k % max(1, o["log_freq"]) == 0 or k == o["iterations"] - 1]
-    print("Logs | Iter | PSNR", file=out)
-    for k, p in zip(its, logs):
-        print(f"Logs | {k} | {p:.2f}", file=out)
+    ssims = list(res["log_ssim"]) if o["ssim_metric"] else [None] * len(logs)
+    print("Logs | Iter | PSNR" + (" | SSIM" if o["ssim_metric"] else ""), file=out)
+    for k, p, q in zip(its, logs, ssims):
+        print(f"Logs | {k} | {p:.2f}" + (f" | {q:.4f}" if q is not None else ""), file=out)
     os.makedirs(run["out_dir"], exist_ok=True)
     stem = os.path.splitext(os.path.basename(run["target"]))[0]
     img_path = os.path.join(run["out_dir"], stem + ".png")
@@ -216,6 +215,8 @@ def train(run: dict, out=None) -> dict:
     sites_io.write_sites_txt(final, w, h, sites_path)
     print(f"Saved: {sites_path}", file=out)
     print(f"Final PSNR: {res['final_psnr']} dB (best {res['best_psnr']})", file=out)
+    if o["ssim_metric"]:
+        print(f"Final SSIM: {res['final_ssim']} (best {res['best_ssim']})", file=out)
     print(f"Training time: {res['train_seconds']:.2f} s", file=out)
     print(f"Total time: {time.perf_counter() - t_start:.2f} s", file=out)
     return dict(result=res, image_path=img_path, sites_path=sites_path, n_active=final.shape[0])

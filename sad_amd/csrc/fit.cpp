@@ -177,6 +177,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&d_mask, (size_t)np));
     SAD_TRY(mem.alloc(&d_render, (size_t)np * 3));
     SAD_TRY(mem.alloc(&d_mse, 1));
+    float* d_ssim;
+    SAD_TRY(mem.alloc(&d_ssim, 1));
     SAD_TRY(mem.alloc(&d_count, 1));
     SAD_TRY(mem.alloc(&d_sites, cap));
     SAD_TRY(mem.alloc(&d_adam, cap));
@@ -254,7 +256,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     uint32_t jump_index = 0;
     bool hil_ready = false;
     uint32_t hil_count = 0;
-    float best = 0.0f, final_psnr = 0.0f;
+    float best = 0.0f, final_psnr = 0.0f, best_ssim = 0.0f, final_ssim = 0.0f;
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
 
@@ -342,8 +344,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
             gev.push_back(b);
             HIP_TRY(hipEventRecord(a, S));
         }
+        /* the SSIM loss term reads a render of the current state (sad_cuda.cu:2307-2318) */
+        if (o.ssim_weight > 0.0f) SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
         SAD_TRY(launch_gradients_aos(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width, height,
-                                     cw, ch, S));
+                                     cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             gbytes.push_back(48.0 * np + 120.0 * n + (prune ? 8.0 * n : 0.0));
@@ -376,14 +380,22 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         if (iter % std::max(1, o.log_freq) == 0 || iter == o.iterations - 1) {
             SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
             SAD_TRY(launch_sse(d_render, d_target, d_mask, d_mse, width, height, S));
+            if (o.ssim_metric) SAD_TRY(launch_ssim(d_render, d_target, d_mask, d_ssim, width, height, S));
             SAD_TRY(launch_count_active(d_sites, d_count, n, S));
-            float sse = 0.0f;
+            float sse = 0.0f, ssim_sum = 0.0f;
             HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
+            if (o.ssim_metric) HIP_TRY(hipMemcpyAsync(&ssim_sum, d_ssim, sizeof(float), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipMemcpyAsync(&active_count, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipStreamSynchronize(S));
             float mse = sse / (std::max(1.0f, mask_sum) * 3.0f);
             float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
             if (log_psnr && n_logs < log_cap) log_psnr[n_logs] = psnr;
+            if (o.ssim_metric) {
+                const float ssim = ssim_sum / std::max(1.0f, mask_sum);   /* sad_cuda.cu:2443-2448 */
+                if (res && res->log_ssim && n_logs < log_cap) res->log_ssim[n_logs] = ssim;
+                final_ssim = ssim;
+                best_ssim = std::max(best_ssim, ssim);
+            }
             n_logs++;
             final_psnr = psnr;
             best = std::max(best, psnr);
@@ -412,6 +424,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     if (res) {
         res->final_psnr = final_psnr;
         res->best_psnr = best;
+        res->final_ssim = final_ssim;
+        res->best_ssim = best_ssim;
         res->actual_sites = actual;
         res->active_estimate = active_est;
         res->active_count = (int)active_count;

@@ -19,7 +19,8 @@ namespace sadh {
 int launch_render(const uint32_t*, const uint32_t*, float*, const SadSite*, float, uint32_t, int, int, int, int,
                   hipStream_t);
 int launch_gradients(const uint32_t*, const uint32_t*, const float*, const float*, float* const[10], const SadSite*,
-                     float, uint32_t, float*, bool, int, int, int, int, hipStream_t);
+                     float, uint32_t, float*, bool, int, int, int, int, hipStream_t, const float* rendered = nullptr,
+                     float ssim_weight = 0.0f);
 int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
                       float* const[8], int, int, int, int, hipStream_t);
 int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
@@ -98,7 +99,10 @@ struct StreamPair {
 
 /* AoS [N][16] gradient lines (the fit's accumulation layout) */
 int launch_gradients_aos(const uint32_t*, const uint32_t*, const float*, const float*, float*, const SadSite*, float,
-                         uint32_t, float*, bool, int, int, int, int, hipStream_t);
+                         uint32_t, float*, bool, int, int, int, int, hipStream_t, const float* rendered = nullptr,
+                         float ssim_weight = 0.0f);
+int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
+                hipStream_t stream);
 int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
                         SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
                         uint32_t, int, int, hipStream_t);

@@ -163,9 +163,12 @@ struct PixelArgs {
     const SadSite* sites;
     const float* target;       /* GRAD, STATS */
     const float* mask;         /* GRAD, STATS */
+    const float* rendered;     /* GRAD with ssim_weight > 0: the render of the current sites */
+    float ssim_weight;         /* GRAD: weight of the -SSIM loss term */
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3] */
-    float* g[10];              /* value j of site i accumulates at g[j] + i * gstride */
+    float* g[10];              /* value j of site i accumulates at g[j] + i * gstride (LINES = false) */
+    float* gline;              /* LINES = true: value j of site i accumulates at gline + i * gstride + j */
     uint32_t gstride;          /* 1: SoA arrays; 10: AoS [N][10] (BWD); 16: AoS [N][16] lines */
     float* removal;            /* GRAD with removal */
     float iss, inv_scale;
@@ -178,8 +181,17 @@ template <int MODE, bool REMOVAL, int H>
 using PixelLds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>,
                                            TileLds<H, REMOVAL, NumValues<MODE, REMOVAL>::value>>::type;
 
+/* address of value j of site id: AoS lines index by arithmetic, so a PixelArgs
+ * built in registers is never indexed by a lane-varying value */
+template <bool LINES>
+__device__ __forceinline__ float* grad_addr(const PixelArgs& A, uint32_t j, uint32_t id)
+{
+    if constexpr (LINES) return A.gline + (size_t)id * A.gstride + j;
+    else return A.g[j] + (size_t)id * A.gstride;
+}
+
 /* One 16x16 tile (index `tile` of A's image) of any mode; 256 threads, one pixel each. */
-template <int MODE, bool REMOVAL, int H>
+template <int MODE, bool REMOVAL, int H, bool LINES>
 __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t tile, PixelLds<MODE, REMOVAL, H>& L)
 {
     constexpr int NV = NumValues<MODE, REMOVAL>::value;
@@ -313,6 +325,14 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             float dr = pr - tr, dg = pgr - tg, db = pb - tb;
             gr = 2.0f * dr; gg = 2.0f * dg; gb = 2.0f * db;
             err = dr * dr + dg * dg + db * db;
+            if (MODE == MODE_GRAD && A.ssim_weight > 0.0f) {
+                /* -w SSIM of the pixel's 3x3 window, through its own luma (gradients.cu:236-266) */
+                SadSsimStats st = sad_ssim_window(A.rendered, A.target, A.width, A.height, x, y);
+                float f = -A.ssim_weight * sad_ssim_dcentre(st, sad_luma3(A.rendered + 3u * pix), sad_luma3(tp));
+                gr += f * 0.299f;
+                gg += f * 0.587f;
+                gb += f * 0.114f;
+            }
         }
     }
     L.px[tid] = make_float4(pr, pgr, pb, err);
@@ -397,7 +417,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
                 if (v != 0.0f) {
                     const uint32_t id = L.start[e];
                     if (MODE == MODE_GRAD && REMOVAL && j == 10u) unsafeAtomicAdd(A.removal + id, v);
-                    else unsafeAtomicAdd(A.g[j] + id * A.gstride, v);
+                    else unsafeAtomicAdd(grad_addr<LINES>(A, j, id), v);
                 }
             }
             __syncthreads();
@@ -516,17 +536,17 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
             if (v != 0.0f) unsafeAtomicAdd(A.removal + id, v);
         } else {
-            unsafeAtomicAdd(A.g[j] + id * A.gstride, v);
+            unsafeAtomicAdd(grad_addr<LINES>(A, j, id), v);
         }
     }
     }  /* backward modes */
 }
 
-template <int MODE, bool REMOVAL, int H>
+template <int MODE, bool REMOVAL, int H, bool LINES>
 __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
     __shared__ PixelLds<MODE, REMOVAL, H> L;
-    pixel_tile<MODE, REMOVAL, H>(A, xcd_tile(blockIdx.x, A.ntiles), L);
+    pixel_tile<MODE, REMOVAL, H, LINES>(A, xcd_tile(blockIdx.x, A.ntiles), L);
 }
 
 /* Batched fit: one launch over the concatenated tiles of every job. */
@@ -544,8 +564,7 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.sites = J.sites;
     A.target = J.target;
     A.mask = J.mask;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) A.g[k] = J.gacc + k;
+    A.gline = J.gacc;
     A.gstride = 16u;
     A.removal = J.removal;
     A.iss = J.iss;
@@ -557,7 +576,7 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.ch = J.ch;
     A.tiles_x = J.tiles_x;
     A.ntiles = J.ntiles;
-    pixel_tile<MODE_GRAD, REMOVAL, H>(A, f - J.tile_begin, L);
+    pixel_tile<MODE_GRAD, REMOVAL, H, true>(A, f - J.tile_begin, L);
 }
 
 /* ------------------------------------------------------------ launchers -- */
@@ -567,13 +586,13 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
 #endif
 constexpr int HASH_SLOTS = SAD_HASH_SLOTS;
 
-template <int MODE, bool REMOVAL>
+template <int MODE, bool REMOVAL, bool LINES = false>
 static int launch_pixel(PixelArgs& A, hipStream_t stream, const char* what)
 {
     A.tiles_x = div_up((uint32_t)A.width, 16u);
     A.ntiles = A.tiles_x * div_up((uint32_t)A.height, 16u);
     A.inv_scale = sqrtf(A.iss);
-    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS>), dim3(A.ntiles), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS, LINES>), dim3(A.ntiles), dim3(256), 0, stream, A);
     return launch_status(what);
 }
 
@@ -588,10 +607,12 @@ int launch_render(const uint32_t* cand0, const uint32_t* cand1, float* out, cons
 
 int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                      float* const g[10], const SadSite* sites, float iss, uint32_t n, float* removal,
-                     bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream)
+                     bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
+                     const float* rendered, float ssim_weight)
 {
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
+    A.rendered = rendered; A.ssim_weight = ssim_weight;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
     A.gstride = 1u;
     A.removal = removal; A.iss = iss; A.n = n;
@@ -603,16 +624,18 @@ int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* 
 /* gradients into AoS [N][16] lines (values 0-9 in Site field order) */
 int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                          float* gacc, const SadSite* sites, float iss, uint32_t n, float* removal, bool compute_removal,
-                         int width, int height, int cw, int ch, hipStream_t stream)
+                         int width, int height, int cw, int ch, hipStream_t stream, const float* rendered,
+                         float ssim_weight)
 {
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
-    for (int j = 0; j < 10; ++j) A.g[j] = gacc + j;
+    A.rendered = rendered; A.ssim_weight = ssim_weight;
+    A.gline = gacc;
     A.gstride = 16u;
     A.removal = removal; A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    if (compute_removal) return launch_pixel<MODE_GRAD, true>(A, stream, "compute_gradients_tiled(aos)");
-    return launch_pixel<MODE_GRAD, false>(A, stream, "compute_gradients_tiled(aos)");
+    if (compute_removal) return launch_pixel<MODE_GRAD, true, true>(A, stream, "compute_gradients_tiled(aos)");
+    return launch_pixel<MODE_GRAD, false, true>(A, stream, "compute_gradients_tiled(aos)");
 }
 
 int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
@@ -666,11 +689,11 @@ int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const flo
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_backward: bad sizes");
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.grad_out = grad_output;
-    for (int j = 0; j < 10; ++j) A.g[j] = grad_sites + j;
+    A.gline = grad_sites;
     A.gstride = 10u;
     A.iss = inv_scale_sq; A.n = site_count; A.width = width; A.height = height;
     A.cw = cand_width; A.ch = cand_height;
-    return launch_pixel<MODE_BWD, false>(A, to_hip(stream), "render_backward");
+    return launch_pixel<MODE_BWD, false, true>(A, to_hip(stream), "render_backward");
 }
 
 int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, const float* target,
@@ -681,9 +704,8 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, c
                                  float* removal_delta, uint32_t compute_removal, float ssim_weight, int width,
                                  int height, int cand_width, int cand_height, sadh_stream_t stream)
 {
-    (void)rendered;
-    if (ssim_weight > 0.0f) return fail(SADH_ERR_UNSUPPORTED, "compute_gradients_tiled: SSIM loss is not implemented");
     SADH_REQUIRE(cand0 && cand1 && target && mask && sites, "compute_gradients_tiled: null pointer");
+    SADH_REQUIRE(!(ssim_weight > 0.0f) || rendered, "compute_gradients_tiled: the SSIM term needs `rendered`");
     SADH_REQUIRE(grad_pos_x && grad_pos_y && grad_log_tau && grad_radius && grad_color_r && grad_color_g &&
                      grad_color_b && grad_dir_x && grad_dir_y && grad_log_aniso,
                  "compute_gradients_tiled: null gradient array");
@@ -692,7 +714,8 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, c
     float* const g[10] = {grad_pos_x, grad_pos_y, grad_log_tau, grad_radius, grad_color_r,
                           grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
     return launch_gradients(cand0, cand1, target, mask, g, sites, inv_scale_sq, site_count, removal_delta,
-                            compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream));
+                            compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream), rendered,
+                            ssim_weight > 0.0f ? ssim_weight : 0.0f);
 }
 
 int sadh_compute_gradients_tiled_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target,
@@ -705,7 +728,8 @@ int sadh_compute_gradients_tiled_aos(const uint32_t* cand0, const uint32_t* cand
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0,
                  "compute_gradients_tiled_aos: bad sizes");
     return launch_gradients_aos(cand0, cand1, target, mask, grad_lines, sites, inv_scale_sq, site_count, removal_delta,
-                                compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream));
+                                compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream), nullptr,
+                                0.0f);
 }
 
 int sadh_compute_site_stats_simple(const uint32_t* cand0, const uint32_t* cand1, const float* target,

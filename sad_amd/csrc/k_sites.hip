@@ -479,6 +479,26 @@ __global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const 
     if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
+/* computeSSIMKernel, utils.cu:29-87: sum over masked pixels of the SSIM of the
+ * pixel's 3x3 luma window; 16x16 tiles so the windows overlap in cache */
+__global__ __launch_bounds__(256) void k_ssim(const float* __restrict__ r, const float* __restrict__ t,
+                                              const float* __restrict__ mask, float* __restrict__ acc, int width,
+                                              int height, uint32_t tiles_x)
+{
+    __shared__ float part[4];
+    const int x = (int)(blockIdx.x % tiles_x) * 16 + (int)(threadIdx.x & 15u);
+    const int y = (int)(blockIdx.x / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+    float v = 0.0f;
+    if (x < width && y < height && mask[(size_t)y * width + x] > 0.0f) {
+        float A, B, C, D;
+        v = sad_ssim_terms(sad_ssim_window(r, t, width, height, x, y), &A, &B, &C, &D);
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
+}
+
 /* countActiveSitesKernel, utils.cu:90-101 */
 __global__ __launch_bounds__(256) void k_count_active(const SadSite* __restrict__ sites, uint32_t* count, uint32_t n)
 {
@@ -766,6 +786,15 @@ int launch_sse(const float* rendered, const float* target, const float* mask, fl
     return launch_status("compute_psnr");
 }
 
+int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
+                hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
+    const uint32_t tx = div_up((uint32_t)width, 16u), ty = div_up((uint32_t)height, 16u);
+    hipLaunchKernelGGL(k_ssim, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width, height, tx);
+    return launch_status("compute_ssim");
+}
+
 int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, count);
@@ -985,6 +1014,13 @@ int sadh_compute_psnr(const float* rendered, const float* target, const float* m
 {
     SADH_REQUIRE(rendered && target && mask && mse_accum && width > 0 && height > 0, "compute_psnr: bad arguments");
     return launch_sse(rendered, target, mask, mse_accum, width, height, to_hip(stream));
+}
+
+int sadh_compute_ssim(const float* rendered, const float* target, const float* mask, float* ssim_accum, int width,
+                      int height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(rendered && target && mask && ssim_accum && width > 0 && height > 0, "compute_ssim: bad arguments");
+    return launch_ssim(rendered, target, mask, ssim_accum, width, height, to_hip(stream));
 }
 
 int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count, sadh_stream_t stream)

@@ -443,6 +443,71 @@ SAD_HD int sad_hilbert_bits(int width, int height)
     return bits > 1 ? bits : 1;
 }
 
+/* ---------------------------------------------------------------- SSIM --
+ * 3x3 window statistics of the luma (0.299, 0.587, 0.114) of a rendered and a
+ * target image, the window clamped at the borders (computeSSIMKernel,
+ * utils.cu:29-87; ssim_window_stats_shared, gradients.cu:44-78), with
+ * SSIM_C1 = 1e-4, SSIM_C2 = 9e-4 and 1/9 as 0.111111111f (sad_common.cuh:16-18). */
+#define SAD_SSIM_C1 0.0001f
+#define SAD_SSIM_C2 0.0009f
+#define SAD_SSIM_INV_N 0.111111111f
+
+typedef struct SadSsimStats {
+    float mu_x, mu_y, ex2, ey2, exy;
+} SadSsimStats;
+
+SAD_HD float sad_luma3(const float* p) { return fmaf(0.114f, p[2], fmaf(0.587f, p[1], 0.299f * p[0])); }
+
+SAD_HD SadSsimStats sad_ssim_window(const float* r, const float* t, int w, int h, int x, int y)
+{
+    SadSsimStats s = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = sad_clampi(y + dy, 0, h - 1);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const size_t i = 3 * ((size_t)yy * (size_t)w + (size_t)sad_clampi(x + dx, 0, w - 1));
+            const float xv = sad_luma3(r + i), yv = sad_luma3(t + i);
+            s.mu_x += xv;
+            s.mu_y += yv;
+            s.ex2 = fmaf(xv, xv, s.ex2);
+            s.ey2 = fmaf(yv, yv, s.ey2);
+            s.exy = fmaf(xv, yv, s.exy);
+        }
+    }
+    s.mu_x *= SAD_SSIM_INV_N;
+    s.mu_y *= SAD_SSIM_INV_N;
+    s.ex2 *= SAD_SSIM_INV_N;
+    s.ey2 *= SAD_SSIM_INV_N;
+    s.exy *= SAD_SSIM_INV_N;
+    return s;
+}
+
+/* SSIM of a window = (A B) / max(C D, 1e-8) with the four factors out */
+SAD_HD float sad_ssim_terms(SadSsimStats s, float* A, float* B, float* C, float* D)
+{
+    const float sx2 = fmaxf(fmaf(-s.mu_x, s.mu_x, s.ex2), 0.0f);
+    const float sy2 = fmaxf(fmaf(-s.mu_y, s.mu_y, s.ey2), 0.0f);
+    const float sxy = fmaf(-s.mu_x, s.mu_y, s.exy);
+    *A = fmaf(2.0f * s.mu_x, s.mu_y, SAD_SSIM_C1);
+    *B = fmaf(2.0f, sxy, SAD_SSIM_C2);
+    *C = fmaf(s.mu_y, s.mu_y, s.mu_x * s.mu_x) + SAD_SSIM_C1;
+    *D = (sx2 + sy2) + SAD_SSIM_C2;
+    return (*A * *B) / fmaxf(*C * *D, 1e-8f);
+}
+
+/* d SSIM(window of pixel k) / d x_k, the window centre's own rendered luma,
+ * other pixels held fixed (gradients.cu:236-262); yk is the target luma there */
+SAD_HD float sad_ssim_dcentre(SadSsimStats s, float xk, float yk)
+{
+    float A, B, C, D;
+    (void)sad_ssim_terms(s, &A, &B, &C, &D);
+    const float denom = fmaxf(C * D, 1e-8f), num = A * B;
+    const float dmu = SAD_SSIM_INV_N;
+    const float dsig = (2.0f * SAD_SSIM_INV_N) * (xk - s.mu_x);
+    const float dcov = SAD_SSIM_INV_N * (yk - s.mu_y);
+    const float dA = 2.0f * s.mu_y * dmu, dB = 2.0f * dcov, dC = 2.0f * s.mu_x * dmu, dD = dsig;
+    return (dA * B + A * dB) / denom - num * (dC * D + C * dD) / (denom * denom);
+}
+
 /* ------------------------------------------- gradient-weighted init ----
  * Seed s of the R2 low-discrepancy sequence (g = plastic number) lands at
  * uv = frac(0.5 + s * (1/g, 1/g^2)), pixel position uv * (W-1, H-1).  The

@@ -15,8 +15,8 @@ Differences from the reference wrappers:
     stream, so the ops can be captured in HIP graphs and run on side streams;
   * `init_gradient_weighted` hands out the R2 seeds in site order, where the
     reference's threads race for them (same sites, deterministic order);
-  * `compute_gradients_tiled` with ssim_weight > 0 fails with the library's
-    "unsupported" status (the SSIM term is not implemented).
+  * `compute_gradients_tiled` with ssim_weight > 0 checks that `rendered` has
+    the target's size (the SSIM term reads it at every pixel).
 """
 from __future__ import annotations
 
@@ -348,6 +348,7 @@ def compute_gradients_tiled(cand0, cand1, target, rendered, mask, sites, grad_po
     _cuda(removal_delta, "removal_delta")
     _check(removal_delta.is_contiguous(), "removal_delta must be contiguous")
     _check(cand0.size(0) == cand_width * cand_height, "cand0 size must match cand_width*cand_height")
+    _check(not ssim_weight > 0.0 or rendered.shape == target.shape, "rendered size must match target")
     _run("sadh_compute_gradients_tiled", cand0, _p(cand0), _p(cand1), _p(target), _p(rendered), _p(mask),
          *[_p(g) for g in grads], _p(sites), float(inv_scale_sq), site_count, _p(removal_delta),
          int(compute_removal), float(ssim_weight), target.size(1), target.size(0), cand_width, cand_height)

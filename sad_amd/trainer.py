@@ -48,6 +48,8 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
     render = np.zeros((h, w, 3), np.float32) if want_render else None
     if render is not None:
         res.final_render = render.ctypes.data_as(C.POINTER(C.c_float))
+    slogs = np.zeros(len(logs), np.float32)
+    res.log_ssim = slogs.ctypes.data_as(C.POINTER(C.c_float))
     s = to_struct(o)
     _lib.call("sadh_fit", C.byref(s), _lib.ptr(tgt), _lib.ptr(msk), w, h, _lib.ptr(buf), cap, _lib.ptr(logs),
               len(logs), C.byref(res))
@@ -56,6 +58,7 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
     out["render"] = render
     out["sites"] = buf
     out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
+    out["log_ssim"] = slogs[: min(res.n_logs, len(logs))].copy()
     out["plan"] = plan
     return out
 

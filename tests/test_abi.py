@@ -32,11 +32,14 @@ def test_null_pointers_are_rejected(name, args):
     assert "null" in lib.sadh_last_error().decode() or "bad" in lib.sadh_last_error().decode()
 
 
-def test_ssim_loss_is_refused():
+def test_ssim_loss_needs_rendered():
+    """The SSIM term reads `rendered`: a null one is refused before any launch."""
     lib = _lib.load()
     z = C.c_void_p(16)
-    rc = lib.sadh_compute_gradients_tiled(*([z] * 15), z, 1.0, 1, z, 0, 0.5, 2, 2, 2, 2, None)
-    assert rc == -2 and "SSIM" in lib.sadh_last_error().decode()
+    args = [z] * 15
+    args[3] = None                                   # rendered
+    rc = lib.sadh_compute_gradients_tiled(*args, z, 1.0, 1, z, 0, 0.5, 2, 2, 2, 2, None)
+    assert rc == -1 and "rendered" in lib.sadh_last_error().decode()
 
 
 def test_fit_plan_and_capacity_error():

@@ -142,8 +142,8 @@ def test_cli_parse_defaults_and_flags():
         with pytest.raises(cli.UsageError):
             cli.parse_args(bad, cfg)
     assert cli.parse_args(["-h"], cfg)["mode"] == "help"
-    with pytest.raises(cli.UsageError):
-        cli.train(cli.parse_args(["x.png", "--ssim"], cfg))
+    o = cli.parse_args(["x.png", "--ssim", "--ssim-weight", "0.25"], cfg)["opts"]
+    assert o["ssim_metric"] == 1 and o["ssim_weight"] == 0.25
     assert cli.main(["--bogus"]) == 1
 
 
@@ -158,11 +158,11 @@ def test_cli_fit_and_render(tmp_path, capsys):
     sites_io.save_image(str(img), t)
     out = tmp_path / "out"
     rc = cli.main([str(img), "--iters", "60", "--sites", "600", "--target-bpp", "1.5", "--log-freq", "20",
-                   "--out-dir", str(out)])
+                   "--out-dir", str(out), "--ssim"])
     assert rc == 0
     log = capsys.readouterr().out
     print(log)
-    assert "Final PSNR" in log and "init=gradient" in log
+    assert "Final PSNR" in log and "init=gradient" in log and "Final SSIM" in log
     png, txt = out / "scene.png", out / "scene_sites.txt"
     assert png.exists() and txt.exists()
     sites, w, h = sites_io.load_sites(str(txt))

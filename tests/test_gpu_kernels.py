@@ -213,13 +213,6 @@ def test_gradients_parity(gpu, orc, w, h, n, passes, removal):
         assert _rel_ok(gpu.get(rd), racc, rab) <= 1.0
 
 
-def test_gradients_reject_ssim(gpu):
-    from sad_amd._lib import SadHipError
-    z = gpu.zeros((16,))
-    with pytest.raises(SadHipError, match="SSIM"):
-        gpu.call("sadh_compute_gradients_tiled", z, z, z, z, z, *([z] * 10), z, 1.0, 1, z, 0, 0.5, 2, 2, 2, 2)
-
-
 @pytest.mark.parametrize("removal", [False, True])
 @pytest.mark.parametrize("w,h,n,passes", [(160, 128, 900, 12), (256, 256, 2048, 0)])
 def test_gradients_aos_parity(gpu, orc, w, h, n, passes, removal):
