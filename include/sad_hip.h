@@ -237,6 +237,20 @@ int sadh_compute_psnr(const float* rendered, const float* target, const float* m
 int sadh_compute_ssim(const float* rendered, const float* target, const float* mask,
                       float* ssim_accum, int width, int height, sadh_stream_t stream);
 
+/* makePackedInferenceSites, backends/metal/sources/utils/packed_inference.swift:86-199,
+ * on the device: `quant` (device, one struct) receives the ranges of log_tau,
+ * radius and colour over the active sites, `packed` one 16-byte site per input
+ * site (inactive ones flagged).  The anisotropy angle uses this library's
+ * atan2 (sad_math.h); Swift's is not reproduced bit for bit. */
+int sadh_pack_inference_sites(const SadSite* sites, uint32_t site_count, int width, int height,
+                              SadInferenceSite* packed, SadInferenceQuant* quant, sadh_stream_t stream);
+
+/* decodePackedSite, backends/metal/shaders/sad_common.metal:71-120: back to
+ * Site form (device arrays); an inactive site decodes with pos = (-1, -1). */
+int sadh_unpack_inference_sites(const SadInferenceSite* packed, const SadInferenceQuant* quant,
+                                uint32_t site_count, int width, int height, SadSite* sites,
+                                sadh_stream_t stream);
+
 /* launchCountActiveSites, utils.cu:637-651 */
 int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count,
                             sadh_stream_t stream);
@@ -285,6 +299,8 @@ typedef struct SadhFitOptions {
     float ssim_weight;                /* > 0: loss L2 - w * SSIM, with a render before each gradient
                                          (--ssim-weight, sad_cuda.cu:2307-2330); sadh_fit only */
     int ssim_metric;                  /* 1: log the mean SSIM beside the PSNR (--ssim, sad_cuda.cu:2432-2447) */
+    int packed_psnr;                  /* 1: after the fit, render the quantised inference sites with the final
+                                         candidates and report that PSNR (--packed-psnr, training.swift:676-703) */
 } SadhFitOptions;
 
 typedef struct SadhFitResult {
@@ -303,6 +319,8 @@ typedef struct SadhFitResult {
                                          final candidates (sad_cuda.cu:2482-2496); sadh_fit only */
     float final_ssim, best_ssim;      /* ssim_metric: mean SSIM at the last log, and the best */
     float* log_ssim;                  /* in: optional host array of log_cap entries; out: the logged SSIMs */
+    float packed_psnr;                /* packed_psnr: PSNR of the quantised inference sites */
+    float* packed_render;             /* in: optional host [H][W][3]; out: that render (packed_psnr) */
 } SadhFitResult;
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does
@@ -390,6 +408,15 @@ int sadh_render_sites(const SadSite* sites, uint32_t site_count, int width, int 
                       float radius_scale, uint32_t radius_probes, uint32_t inject_count,
                       uint32_t hilbert_window, uint32_t hilbert_probes, float* output,
                       float* cand_ms, float* render_ms);
+
+/* renderPackedInference, backends/metal/sources/rendering/packed_renderer.swift:15-137
+ * (render mode --render-packed): the render job of sadh_render_sites on
+ * quantised inference sites (host arrays), decoded on the device first. */
+int sadh_render_packed_sites(const SadInferenceSite* packed, const SadInferenceQuant* quant,
+                             uint32_t site_count, int width, int height, int cand_passes, int use_jfa,
+                             int jfa_rounds, int cand_downscale, float radius_scale,
+                             uint32_t radius_probes, uint32_t inject_count, uint32_t hilbert_window,
+                             uint32_t hilbert_probes, float* output, float* cand_ms, float* render_ms);
 
 #ifdef __cplusplus
 }

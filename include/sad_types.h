@@ -8,6 +8,7 @@
  *            contract allocates (backends/pytorch/train.py:836,918).
  * PackedSite 8 x fp16 (16 B): sad_common.cuh:34-41.
  * Pair       (key, value) uint32 pair sorted by key (the reference's uint2).
+ * InferenceSite / InferenceQuant: the 16-byte quantised inference format (Metal backend).
  */
 #ifndef SAD_TYPES_H
 #define SAD_TYPES_H
@@ -49,5 +50,24 @@ typedef struct SadPair {
     uint32_t key;
     uint32_t val;
 } SadPair;
+
+/* 16-byte quantised inference site (backends/metal/sources/utils/packed_inference.swift:3-18,
+ * shaders/sad_common.metal:18-40):
+ *   w[0] = pos.x unorm15 | pos.y unorm15 << 15 | inactive << 30   (over 0..W-1, 0..H-1)
+ *   w[1] = r unorm11 | g unorm11 << 11 | b unorm10 << 22          (over the colour ranges)
+ *   w[2] = log_tau unorm16 | radius unorm16 << 16                 (over their ranges)
+ *   w[3] = angle of aniso_dir unorm16 over [-pi, pi] | fp16 log_aniso << 16 */
+typedef struct SadInferenceSite {
+    uint32_t w[4];
+} SadInferenceSite;
+
+/* ranges of the quantised fields over the active sites: value = min + q * scale */
+typedef struct SadInferenceQuant {
+    float log_tau_min, log_tau_scale;
+    float radius_min, radius_scale;
+    float r_min, r_scale;
+    float g_min, g_scale;
+    float b_min, b_scale;
+} SadInferenceQuant;
 
 #endif /* SAD_TYPES_H */

@@ -345,3 +345,32 @@ def ssim_dcentre(rendered, target, mask=None):
     out = np.zeros((h, w), np.float32)
     lib().orc_ssim_dcentre(_p(r), _p(t), _p(m), I32(w), I32(h), _p(out))
     return out
+
+
+QUANT_FIELDS = ("log_tau_min", "log_tau_scale", "radius_min", "radius_scale", "r_min", "r_scale", "g_min", "g_scale",
+                "b_min", "b_scale")
+
+
+def pack_inference(sites, width, height):
+    """-> (packed uint32 [N][4], quant float32 [10] in QUANT_FIELDS order)"""
+    s = _f32(sites)
+    n = s.shape[0]
+    out = np.zeros((n, 4), np.uint32)
+    q = np.zeros(10, np.float32)
+    lib().orc_pack_inference(_p(s), U32(n), I32(width), I32(height), _p(out), _p(q))
+    return out, q
+
+
+def unpack_inference(packed, quant, width, height):
+    p, q = _u32(packed), _f32(quant)
+    n = p.shape[0]
+    out = np.zeros((n, 10), np.float32)
+    lib().orc_unpack_inference(_p(p), _p(q), U32(n), I32(width), I32(height), _p(out))
+    return out
+
+
+def atan2(y, x):
+    y, x = _f32(y), _f32(x)
+    out = np.zeros_like(y)
+    lib().orc_atan2_array(_p(y), _p(x), _p(out), I64(y.size))
+    return out

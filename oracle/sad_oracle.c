@@ -876,3 +876,36 @@ ORC_API int orc_init_grad_eval(const float* target, const float* mask, int width
     out4[0] = x; out4[1] = y; out4[2] = (float)p; out4[3] = g;
     return ok;
 }
+
+/* makePackedInferenceSites, backends/metal/sources/utils/packed_inference.swift:86-199:
+ * ranges over active sites (both coordinates >= 0, finite values), then each
+ * site packed with sad_pack_inference (sad_math.h). */
+ORC_API void orc_pack_inference(const SadSite* sites, uint32_t n, int width, int height, SadInferenceSite* out,
+                                SadInferenceQuant* quant)
+{
+    float mn[5], mx[5];
+    for (int k = 0; k < 5; ++k) { mn[k] = INFINITY; mx[k] = -INFINITY; }
+    for (uint32_t i = 0; i < n; ++i) {
+        const SadSite* s = &sites[i];
+        if (!(s->pos_x >= 0.0f && s->pos_y >= 0.0f)) continue;
+        const float v[5] = {s->log_tau, s->radius, s->col_r, s->col_g, s->col_b};
+        for (int k = 0; k < 5; ++k)
+            if (isfinite(v[k])) { mn[k] = fminf(mn[k], v[k]); mx[k] = fmaxf(mx[k], v[k]); }
+    }
+    sad_quant_finish(mn, mx, quant);
+    const float maxx = (float)(width > 1 ? width - 1 : 0), maxy = (float)(height > 1 ? height - 1 : 0);
+    for (uint32_t i = 0; i < n; ++i) out[i] = sad_pack_inference(&sites[i], quant, maxx, maxy);
+}
+
+/* decodePackedSite, backends/metal/shaders/sad_common.metal:71-120 */
+ORC_API void orc_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
+                                  int height, SadSite* out)
+{
+    const float maxx = (float)(width > 1 ? width - 1 : 0), maxy = (float)(height > 1 ? height - 1 : 0);
+    for (uint32_t i = 0; i < n; ++i) out[i] = sad_unpack_inference(in[i], quant, maxx, maxy);
+}
+
+ORC_API void orc_atan2_array(const float* y, const float* x, float* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = sad_atan2f(y[i], x[i]);
+}

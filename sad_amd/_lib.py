@@ -35,7 +35,7 @@ class FitOptions(C.Structure):
         ("lr_scale", F32), ("lr_pos", F32), ("lr_tau", F32), ("lr_radius", F32), ("lr_color", F32),
         ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32),
         ("log_freq", I32), ("profile_gradients", I32), ("init_gradient_alpha", F32), ("init_log_tau", F32),
-        ("init_radius", F32), ("ssim_weight", F32), ("ssim_metric", I32),
+        ("init_radius", F32), ("ssim_weight", F32), ("ssim_metric", I32), ("packed_psnr", I32),
     ]
 
 
@@ -48,12 +48,13 @@ class FitResult(C.Structure):
         ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
         ("grad_bytes", C.c_double), ("grad_ms_trace", C.POINTER(F32)), ("grad_ms_trace_cap", I32),
         ("final_render", C.POINTER(F32)), ("final_ssim", F32), ("best_ssim", F32), ("log_ssim", C.POINTER(F32)),
+        ("packed_psnr", F32), ("packed_render", C.POINTER(F32)),
     ]
 
 
 def fit_result_dict(res: "FitResult") -> dict:
     """SadhFitResult fields as a dict, without its host-pointer fields."""
-    skip = ("grad_ms_trace", "grad_ms_trace_cap", "final_render", "log_ssim")
+    skip = ("grad_ms_trace", "grad_ms_trace_cap", "final_render", "log_ssim", "packed_render")
     return {k: getattr(res, k) for k, _ in FitResult._fields_ if k not in skip}
 
 
@@ -115,6 +116,10 @@ SIGNATURES = {
     "sadh_compute_psnr": [P, P, P, P, I32, I32, P],
     "sadh_count_active_sites": [P, P, U32, P],
     "sadh_compute_ssim": [P, P, P, P, I32, I32, P],
+    "sadh_pack_inference_sites": [P, U32, I32, I32, P, P, P],
+    "sadh_unpack_inference_sites": [P, P, U32, I32, I32, P, P],
+    "sadh_render_packed_sites": [P, P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
+                                 C.POINTER(F32)],
     "sadh_init_gradient_weighted": [P, U32, P, P, P, F32, U32, F32, F32, I32, I32, P],
     "sadh_fit_plan": [C.POINTER(FitOptions), I32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32),
                       C.POINTER(I32)],

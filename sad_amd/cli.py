@@ -55,11 +55,11 @@ Training options:
   --prune-during-densify  --prune-start/-end/-freq N  --prune-percentile F
   --cand-freq N  --cand-passes N  --cand-downscale N  --cand-radius-scale F  --cand-radius-probes N
   --cand-inject N  --cand-hilbert-window N  --cand-hilbert-probes N  --max-sites N  --lr F  --log-freq N
-  --ssim (log SSIM)  --ssim-weight F (loss L2 - F * SSIM)
+  --ssim (log SSIM)  --ssim-weight F (loss L2 - F * SSIM)  --packed-psnr (re-render 16-byte sites, PSNR)
 Render options:
   --out <path>  --width N  --height N  --render-cand-passes N  --render-no-jfa  --render-jfa-rounds N
   --render-cand-downscale N  --render-hilbert-window N  --render-hilbert-probes N  --render-cand-inject N
-  --render-target <path>
+  --render-target <path>  --render-packed (render the 16-byte quantised sites)
 """
 
 
@@ -81,6 +81,7 @@ def parse_args(argv: list, cfg: dict) -> dict:
     run = dict(mode="train", opts=o, target=None, mask=None, init_sites=None, out_dir="results",
                max_dim=int(cfg["MAX_DIM"]),
                render=dict(sites=None, out=None, target=None, width=0, height=0, cand_passes=16, use_jfa=True,
+                           packed=False,
                            jfa_rounds=1, cand_downscale=int(cfg["CAND_DOWNSCALE"]),
                            radius_scale=float(cfg["CAND_RADIUS_SCALE"]),
                            radius_probes=int(cfg["CAND_RADIUS_PROBES"]), inject_count=int(cfg["CAND_INJECT_COUNT"]),
@@ -141,6 +142,12 @@ def parse_args(argv: list, cfg: dict) -> dict:
                     r["target"] = value(a)
                 elif a == "--render-no-jfa":
                     r["use_jfa"] = False
+                elif a == "--render-packed":
+                    if run["mode"] != "render":
+                        raise UsageError("--render-packed is only valid in render mode")
+                    r["packed"] = True
+                elif a == "--packed-psnr":
+                    o["packed_psnr"] = 1
                 else:
                     raise UsageError(f"Unknown flag: {a}")
             elif run["mode"] == "train":
@@ -214,6 +221,11 @@ def train(run: dict, out=None) -> dict:
     final = sites_io.active_sites(res["sites"][: res["actual_sites"]])
     sites_io.write_sites_txt(final, w, h, sites_path)
     print(f"Saved: {sites_path}", file=out)
+    if o["packed_psnr"]:
+        packed_path = os.path.join(run["out_dir"], stem + "_packed.png")
+        sites_io.save_image(packed_path, res["packed_render"])
+        print(f"Saved: {packed_path}", file=out)
+        print(f"Packed inference PSNR: {res['packed_psnr']:.2f} dB", file=out)
     print(f"Final PSNR: {res['final_psnr']} dB (best {res['best_psnr']})", file=out)
     if o["ssim_metric"]:
         print(f"Final SSIM: {res['final_ssim']} (best {res['best_ssim']})", file=out)
@@ -233,11 +245,14 @@ def render(run: dict, out=None) -> dict:
         raise UsageError("TXT sites require --width and --height or a header line with image size.")
     print(f"Loaded {sites.shape[0]} sites", file=out)
     print(f"Render size: {w}x{h}", file=out)
-    job = trainer.render_sites(sites, w, h, cand_passes=max(0, r["cand_passes"]), use_jfa=r["use_jfa"],
-                               jfa_rounds=r["jfa_rounds"], cand_downscale=r["cand_downscale"],
-                               radius_scale=r["radius_scale"], radius_probes=r["radius_probes"],
-                               inject_count=r["inject_count"], hilbert_window=r["hilbert_window"],
-                               hilbert_probes=r["hilbert_probes"])
+    kw = dict(cand_passes=max(0, r["cand_passes"]), use_jfa=r["use_jfa"], jfa_rounds=r["jfa_rounds"],
+              cand_downscale=r["cand_downscale"], radius_scale=r["radius_scale"], radius_probes=r["radius_probes"],
+              inject_count=r["inject_count"], hilbert_window=r["hilbert_window"], hilbert_probes=r["hilbert_probes"])
+    if r["packed"]:
+        packed, quant = trainer.pack_inference(sites, w, h)
+        job = trainer.render_packed_sites(packed, quant, w, h, **kw)
+    else:
+        job = trainer.render_sites(sites, w, h, **kw)
     psnr = None
     if r["target"]:
         t = sites_io.load_image(r["target"], max(w, h))

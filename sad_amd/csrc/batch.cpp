@@ -268,8 +268,8 @@ struct Engine {
     {
         o = opts;
         if (o.init_mode != 0) return fail(SADH_ERR_UNSUPPORTED, "fit_batch: only caller-provided initial sites");
-        if (o.ssim_weight > 0.0f || o.ssim_metric)
-            return fail(SADH_ERR_UNSUPPORTED, "fit_batch: the SSIM loss and metric are sadh_fit only");
+        if (o.ssim_weight > 0.0f || o.ssim_metric || o.packed_psnr)
+            return fail(SADH_ERR_UNSUPPORTED, "fit_batch: the SSIM loss and metric and packed PSNR are sadh_fit only");
         use_hil = o.cand_hilbert_probes > 0 && o.cand_hilbert_window > 0;
         HIP_TRY(hipStreamCreateWithFlags(&sp.main, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&sp.cand, hipStreamNonBlocking));

@@ -416,6 +416,27 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
 
     HIP_TRY(hipMemcpy(host_sites.data(), d_sites, sizeof(SadSite) * cap, hipMemcpyDeviceToHost));
     std::memcpy(sites_io, host_sites.data(), sizeof(SadSite) * cap);
+    float packed_psnr = 0.0f;
+    if (o.packed_psnr) {
+        /* training.swift:676-703: quantise the final sites and render them with the final candidates */
+        SadInferenceSite* d_packed;
+        SadInferenceQuant* d_quant;
+        SadSite* d_dec;
+        SAD_TRY(mem.alloc(&d_packed, std::max(n, 1u)));
+        SAD_TRY(mem.alloc(&d_quant, 1));
+        SAD_TRY(mem.alloc(&d_dec, std::max(n, 1u)));
+        SAD_TRY(launch_pack_inference(d_sites, n, width, height, d_packed, d_quant, S));
+        SAD_TRY(launch_unpack_inference(d_packed, d_quant, n, width, height, d_dec, S));
+        SAD_TRY(launch_render(cA0, cA1, d_render, d_dec, iss, n, width, height, cw, ch, S));
+        SAD_TRY(launch_sse(d_render, d_target, d_mask, d_mse, width, height, S));
+        float sse = 0.0f;
+        HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
+        if (res && res->packed_render)
+            HIP_TRY(hipMemcpyAsync(res->packed_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
+        HIP_TRY(hipStreamSynchronize(S));
+        const float mse = sse / (std::max(1.0f, mask_sum) * 3.0f);
+        packed_psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
+    }
     if (res && res->final_render) {
         SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
         HIP_TRY(hipMemcpyAsync(res->final_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
@@ -426,6 +447,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         res->best_psnr = best;
         res->final_ssim = final_ssim;
         res->best_ssim = best_ssim;
+        res->packed_psnr = packed_psnr;
         res->actual_sites = actual;
         res->active_estimate = active_est;
         res->active_count = (int)active_count;
@@ -444,12 +466,13 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     return SADH_OK;
 }
 
-static int render_impl(const SadSite* sites_h, uint32_t n, int width, int height, int cand_passes, int use_jfa,
-                       int jfa_rounds, int cand_downscale, float radius_scale, uint32_t radius_probes,
-                       uint32_t inject_count, uint32_t hil_window, uint32_t hil_probes, float* out_h, float* cand_ms,
-                       float* render_ms)
+/* sites_h, or packed_h + quant_h (quantised inference sites, decoded on the device first) */
+static int render_impl(const SadSite* sites_h, const SadInferenceSite* packed_h, const SadInferenceQuant* quant_h,
+                       uint32_t n, int width, int height, int cand_passes, int use_jfa, int jfa_rounds,
+                       int cand_downscale, float radius_scale, uint32_t radius_probes, uint32_t inject_count,
+                       uint32_t hil_window, uint32_t hil_probes, float* out_h, float* cand_ms, float* render_ms)
 {
-    if (!sites_h || !out_h || n == 0 || width <= 0 || height <= 0)
+    if (!(sites_h || (packed_h && quant_h)) || !out_h || n == 0 || width <= 0 || height <= 0)
         return fail(SADH_ERR_INVALID, "render_sites: bad arguments");
     const int ds = std::max(1, cand_downscale);
     const int cw = std::max(1, (width + ds - 1) / ds), ch = std::max(1, (height + ds - 1) / ds);
@@ -486,7 +509,20 @@ static int render_impl(const SadSite* sites_h, uint32_t n, int width, int height
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventCreate(&e2));
     struct Ev { hipEvent_t a, b, c; ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); } } evg{e0, e1, e2};
-    HIP_TRY(hipMemcpyAsync(d_sites, sites_h, sizeof(SadSite) * n, hipMemcpyHostToDevice, S));
+    if (sites_h) {
+        HIP_TRY(hipMemcpyAsync(d_sites, sites_h, sizeof(SadSite) * n, hipMemcpyHostToDevice, S));
+    } else {
+        /* renderPackedInference, packed_renderer.swift:15-137: the kernels there decode the
+         * 16-byte sites at every use; the decode is a pure function of the bits, so one
+         * decode pass into Site form feeds the same kernels as the full-precision job */
+        SadInferenceSite* d_packed;
+        SadInferenceQuant* d_quant;
+        SAD_TRY(mem.alloc(&d_packed, n));
+        SAD_TRY(mem.alloc(&d_quant, 1));
+        HIP_TRY(hipMemcpyAsync(d_packed, packed_h, sizeof(SadInferenceSite) * n, hipMemcpyHostToDevice, S));
+        HIP_TRY(hipMemcpyAsync(d_quant, quant_h, sizeof(SadInferenceQuant), hipMemcpyHostToDevice, S));
+        SAD_TRY(launch_unpack_inference(d_packed, d_quant, n, width, height, d_sites, S));
+    }
 
     const int md = std::max(width, height);
     const float iss = 1.0f / ((float)md * (float)md);
@@ -571,9 +607,21 @@ int sadh_render_sites(const SadSite* sites, uint32_t site_count, int width, int 
                       uint32_t inject_count, uint32_t hilbert_window, uint32_t hilbert_probes, float* output,
                       float* cand_ms, float* render_ms)
 {
-    return render_impl(sites, site_count, width, height, cand_passes, use_jfa, jfa_rounds, cand_downscale,
-                       radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes, output, cand_ms,
-                       render_ms);
+    return render_impl(sites, nullptr, nullptr, site_count, width, height, cand_passes, use_jfa, jfa_rounds,
+                       cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
+                       output, cand_ms, render_ms);
+}
+
+int sadh_render_packed_sites(const SadInferenceSite* packed, const SadInferenceQuant* quant, uint32_t site_count,
+                             int width, int height, int cand_passes, int use_jfa, int jfa_rounds, int cand_downscale,
+                             float radius_scale, uint32_t radius_probes, uint32_t inject_count,
+                             uint32_t hilbert_window, uint32_t hilbert_probes, float* output, float* cand_ms,
+                             float* render_ms)
+{
+    if (!packed || !quant) return fail(SADH_ERR_INVALID, "render_packed_sites: null packed sites or quant");
+    return render_impl(nullptr, packed, quant, site_count, width, height, cand_passes, use_jfa, jfa_rounds,
+                       cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
+                       output, cand_ms, render_ms);
 }
 
 }  // extern "C"

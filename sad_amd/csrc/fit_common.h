@@ -103,6 +103,10 @@ int launch_gradients_aos(const uint32_t*, const uint32_t*, const float*, const f
                          float ssim_weight = 0.0f);
 int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
                 hipStream_t stream);
+int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int height, SadInferenceSite* out,
+                          SadInferenceQuant* quant, hipStream_t stream);
+int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
+                            int height, SadSite* out, hipStream_t stream);
 int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
                         SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
                         uint32_t, int, int, hipStream_t);

@@ -499,6 +499,86 @@ __global__ __launch_bounds__(256) void k_ssim(const float* __restrict__ r, const
     if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
+/* -------------------------------------------- quantised inference sites --
+ * makePackedInferenceSites (packed_inference.swift:86-199) on the device: the
+ * field ranges are min/max reductions over active sites, kept as order-
+ * preserving integer images of the floats in the 10 words of the quant
+ * struct (min keys 0-4, max keys 5-9) until one thread turns them into
+ * (min, scale) pairs in place; then one thread per site packs. */
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_quant_init(uint32_t* keys)
+{
+    if (threadIdx.x < 10u) keys[threadIdx.x] = threadIdx.x < 5u ? 0xFFFFFFFFu : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_quant_range(const SadSite* __restrict__ sites, uint32_t n, uint32_t* keys)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t lo[5], hi[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) { lo[k] = 0xFFFFFFFFu; hi[k] = 0u; }
+    if (i < n) {
+        const SadSite st = sites[i];
+        if (st.pos_x >= 0.0f && st.pos_y >= 0.0f) {
+            const float v[5] = {st.log_tau, st.radius, st.col_r, st.col_g, st.col_b};
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (isfinite(v[k])) { lo[k] = f2ord(v[k]); hi[k] = lo[k]; }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
+            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
+        }
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (lo[k] != 0xFFFFFFFFu) atomicMin(keys + k, lo[k]);
+            if (hi[k] != 0u) atomicMax(keys + 5 + k, hi[k]);
+        }
+    }
+}
+
+__global__ void k_quant_finish(SadInferenceQuant* q)
+{
+    if (threadIdx.x != 0u) return;
+    uint32_t* keys = reinterpret_cast<uint32_t*>(q);
+    float mn[5], mx[5];
+    for (int k = 0; k < 5; ++k) {
+        mn[k] = keys[k] == 0xFFFFFFFFu ? INFINITY : ord2f(keys[k]);
+        mx[k] = keys[5 + k] == 0u ? -INFINITY : ord2f(keys[5 + k]);
+    }
+    sad_quant_finish(mn, mx, q);
+}
+
+__global__ __launch_bounds__(256) void k_pack_inference(const SadSite* __restrict__ sites, uint32_t n,
+                                                        const SadInferenceQuant* __restrict__ q, float maxx, float maxy,
+                                                        SadInferenceSite* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = sad_pack_inference(&sites[i], q, maxx, maxy);
+}
+
+__global__ __launch_bounds__(256) void k_unpack_inference(const SadInferenceSite* __restrict__ in, uint32_t n,
+                                                          const SadInferenceQuant* __restrict__ q, float maxx,
+                                                          float maxy, SadSite* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = sad_unpack_inference(in[i], q, maxx, maxy);
+}
+
 /* countActiveSitesKernel, utils.cu:90-101 */
 __global__ __launch_bounds__(256) void k_count_active(const SadSite* __restrict__ sites, uint32_t* count, uint32_t n)
 {
@@ -795,6 +875,30 @@ int launch_ssim(const float* rendered, const float* target, const float* mask, f
     return launch_status("compute_ssim");
 }
 
+int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int height, SadInferenceSite* out,
+                          SadInferenceQuant* quant, hipStream_t stream)
+{
+    uint32_t* keys = reinterpret_cast<uint32_t*>(quant);
+    hipLaunchKernelGGL(k_quant_init, dim3(1), dim3(64), 0, stream, keys);
+    if (n > 0) hipLaunchKernelGGL(k_quant_range, dim3(div_up(n, 256u)), dim3(256), 0, stream, sites, n, keys);
+    hipLaunchKernelGGL(k_quant_finish, dim3(1), dim3(64), 0, stream, quant);
+    const float maxx = (float)std::max(width - 1, 0), maxy = (float)std::max(height - 1, 0);
+    if (n > 0)
+        hipLaunchKernelGGL(k_pack_inference, dim3(div_up(n, 256u)), dim3(256), 0, stream, sites, n, quant, maxx, maxy,
+                           out);
+    return launch_status("pack_inference_sites");
+}
+
+int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
+                            int height, SadSite* out, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    const float maxx = (float)std::max(width - 1, 0), maxy = (float)std::max(height - 1, 0);
+    hipLaunchKernelGGL(k_unpack_inference, dim3(div_up(n, 256u)), dim3(256), 0, stream, in, n, quant, maxx, maxy,
+                       out);
+    return launch_status("unpack_inference_sites");
+}
+
 int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, count);
@@ -1021,6 +1125,20 @@ int sadh_compute_ssim(const float* rendered, const float* target, const float* m
 {
     SADH_REQUIRE(rendered && target && mask && ssim_accum && width > 0 && height > 0, "compute_ssim: bad arguments");
     return launch_ssim(rendered, target, mask, ssim_accum, width, height, to_hip(stream));
+}
+
+int sadh_pack_inference_sites(const SadSite* sites, uint32_t site_count, int width, int height,
+                              SadInferenceSite* packed, SadInferenceQuant* quant, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && packed && quant && width > 0 && height > 0, "pack_inference_sites: bad arguments");
+    return launch_pack_inference(sites, site_count, width, height, packed, quant, to_hip(stream));
+}
+
+int sadh_unpack_inference_sites(const SadInferenceSite* packed, const SadInferenceQuant* quant, uint32_t site_count,
+                                int width, int height, SadSite* sites, sadh_stream_t stream)
+{
+    SADH_REQUIRE(sites && packed && quant && width > 0 && height > 0, "unpack_inference_sites: bad arguments");
+    return launch_unpack_inference(packed, quant, site_count, width, height, sites, to_hip(stream));
 }
 
 int sadh_count_active_sites(const SadSite* sites, uint32_t* count, uint32_t site_count, sadh_stream_t stream)

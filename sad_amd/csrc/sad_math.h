@@ -443,6 +443,129 @@ SAD_HD int sad_hilbert_bits(int width, int height)
     return bits > 1 ? bits : 1;
 }
 
+/* ------------------------------------------------------------- atan2 --
+ * atan(t), |t| <= 1: two half-angle reductions atan(t) = 2 atan(t / (1 +
+ * sqrt(1 + t^2))) bring |t| below tan(pi/16) = 0.199, where the Taylor series
+ * to t^11 truncates below 7e-11.  atan2 by octant symmetry.  Used to quantise
+ * the anisotropy direction of inference sites, where CPU and GPU must agree. */
+#define SAD_PI_F 3.14159274f           /* pi rounded to binary32 */
+
+SAD_HD float sad_atan_unit(float t)
+{
+    float u = t / (1.0f + sqrtf(fmaf(t, t, 1.0f)));
+    u = u / (1.0f + sqrtf(fmaf(u, u, 1.0f)));
+    const float u2 = u * u;
+    float p = -1.0f / 11.0f;
+    p = fmaf(p, u2, 1.0f / 9.0f);
+    p = fmaf(p, u2, -1.0f / 7.0f);
+    p = fmaf(p, u2, 1.0f / 5.0f);
+    p = fmaf(p, u2, -1.0f / 3.0f);
+    p = fmaf(p, u2, 1.0f);
+    return 4.0f * (u * p);
+}
+
+SAD_HD float sad_atan2f(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    float a;
+    if (ax == 0.0f && ay == 0.0f) a = 0.0f;
+    else if (ay <= ax) a = sad_atan_unit(ay / ax);
+    else a = 0.5f * SAD_PI_F - sad_atan_unit(ax / ay);
+    if (signbit(x)) a = SAD_PI_F - a;
+    return signbit(y) ? -a : a;
+}
+
+/* -------------------------------------------- quantised inference sites --
+ * makePackedInferenceSites (backends/metal/sources/utils/packed_inference.swift:86-199)
+ * and decodePackedSite (shaders/sad_common.metal:71-120).  Quantisation:
+ * q = uint(clamp((v - min) / scale, 0, 1) * levels + 0.5); NaN inputs quantise
+ * to 0.  Decoding: v = min + q * (1 / levels) * max(scale, 0). */
+#define SAD_INV_U16 (1.0f / 65535.0f)
+#define SAD_INV_U15 (1.0f / 32767.0f)
+#define SAD_INV_U11 (1.0f / 2047.0f)
+#define SAD_INV_U10 (1.0f / 1023.0f)
+#define SAD_TWO_PI_F 6.28318548f       /* 2 pi rounded to binary32 */
+
+SAD_HD uint32_t sad_unorm_q(float v, float minv, float scale, float levels)
+{
+    if (!(scale > 0.0f)) return 0u;
+    const float n = fminf(fmaxf((v - minv) / scale, 0.0f), 1.0f);
+    return (uint32_t)(n * levels + 0.5f);
+}
+
+SAD_HD uint32_t sad_unorm_q_max(float v, float maxv, float levels)
+{
+    if (!(maxv > 0.0f)) return 0u;
+    const float n = fminf(fmaxf(v / maxv, 0.0f), 1.0f);
+    return (uint32_t)(n * levels + 0.5f);
+}
+
+/* angle of the normalised direction over [-pi, pi] as unorm16 (packAngle) */
+SAD_HD uint32_t sad_pack_angle(float dx, float dy)
+{
+    const float len2 = dx * dx + dy * dy;
+    if (len2 < 1.0e-8f) { dx = 1.0f; dy = 0.0f; }
+    else { const float il = 1.0f / sqrtf(len2); dx = dx * il; dy = dy * il; }
+    const float a = sad_atan2f(dy, dx);
+    const float n = fminf(fmaxf((a + SAD_PI_F) / SAD_TWO_PI_F, 0.0f), 1.0f);
+    return (uint32_t)(n * 65535.0f + 0.5f);
+}
+
+/* one site; maxx, maxy = W-1, H-1 */
+SAD_HD SadInferenceSite sad_pack_inference(const SadSite* s, const SadInferenceQuant* q, float maxx, float maxy)
+{
+    SadInferenceSite o;
+    const int active = s->pos_x >= 0.0f && s->pos_y >= 0.0f;
+    const float px = fminf(fmaxf(s->pos_x, 0.0f), maxx), py = fminf(fmaxf(s->pos_y, 0.0f), maxy);
+    o.w[0] = ((sad_unorm_q_max(px, maxx, 32767.0f) | (sad_unorm_q_max(py, maxy, 32767.0f) << 15)) & 0x3fffffffu) |
+             ((active ? 0u : 1u) << 30);
+    o.w[1] = sad_unorm_q(s->col_r, q->r_min, q->r_scale, 2047.0f) |
+             (sad_unorm_q(s->col_g, q->g_min, q->g_scale, 2047.0f) << 11) |
+             (sad_unorm_q(s->col_b, q->b_min, q->b_scale, 1023.0f) << 22);
+    const float lt = isfinite(s->log_tau) ? s->log_tau : q->log_tau_min;
+    const float rd = isfinite(s->radius) ? s->radius : q->radius_min;
+    o.w[2] = sad_unorm_q(lt, q->log_tau_min, q->log_tau_scale, 65535.0f) |
+             (sad_unorm_q(rd, q->radius_min, q->radius_scale, 65535.0f) << 16);
+    const float la = isfinite(s->log_aniso) ? s->log_aniso : 0.0f;
+    o.w[3] = (sad_pack_angle(s->dir_x, s->dir_y) & 0xffffu) | ((uint32_t)sad_f32_to_f16(la) << 16);
+    return o;
+}
+
+/* decode; an inactive site comes out with pos = (-1, -1), as every kernel here expects */
+SAD_HD SadSite sad_unpack_inference(SadInferenceSite p, const SadInferenceQuant* q, float maxx, float maxy)
+{
+    SadSite s;
+    const uint32_t w0 = p.w[0], w1 = p.w[1], w2 = p.w[2], w3 = p.w[3];
+    s.pos_x = ((float)(w0 & 0x7fffu) * SAD_INV_U15) * fmaxf(maxx, 0.0f);
+    s.pos_y = ((float)((w0 >> 15) & 0x7fffu) * SAD_INV_U15) * fmaxf(maxy, 0.0f);
+    if ((w0 >> 30) & 1u) { s.pos_x = -1.0f; s.pos_y = -1.0f; }
+    s.col_r = q->r_min + ((float)(w1 & 0x7ffu) * SAD_INV_U11) * fmaxf(q->r_scale, 0.0f);
+    s.col_g = q->g_min + ((float)((w1 >> 11) & 0x7ffu) * SAD_INV_U11) * fmaxf(q->g_scale, 0.0f);
+    s.col_b = q->b_min + ((float)((w1 >> 22) & 0x3ffu) * SAD_INV_U10) * fmaxf(q->b_scale, 0.0f);
+    s.log_tau = q->log_tau_min + ((float)(w2 & 0xffffu) * SAD_INV_U16) * fmaxf(q->log_tau_scale, 0.0f);
+    s.radius = q->radius_min + ((float)((w2 >> 16) & 0xffffu) * SAD_INV_U16) * fmaxf(q->radius_scale, 0.0f);
+    const float ang = ((float)(w3 & 0xffffu) * SAD_INV_U16) * SAD_TWO_PI_F - SAD_PI_F;
+    float sn, cs;
+    sad_sincosf(ang, &sn, &cs);
+    s.dir_x = cs;
+    s.dir_y = sn;
+    s.log_aniso = sad_f16_to_f32((uint16_t)(w3 >> 16));
+    return s;
+}
+
+/* the ranges: min and max of the finite values over active sites (pos.x and
+ * pos.y >= 0); a field with none gets (0, 0); scale = max(max - min, 1e-6) */
+SAD_HD void sad_quant_finish(const float mn[5], const float mx[5], SadInferenceQuant* q)
+{
+    float* o = &q->log_tau_min;
+    for (int k = 0; k < 5; ++k) {
+        float a = mn[k], b = mx[k];
+        if (!isfinite(a) || !isfinite(b)) { a = 0.0f; b = 0.0f; }
+        o[2 * k] = a;
+        o[2 * k + 1] = fmaxf(b - a, 1.0e-6f);
+    }
+}
+
 /* ---------------------------------------------------------------- SSIM --
  * 3x3 window statistics of the luma (0.299, 0.587, 0.114) of a rendered and a
  * target image, the window clamped at the borders (computeSSIMKernel,

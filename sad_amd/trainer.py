@@ -50,12 +50,16 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
         res.final_render = render.ctypes.data_as(C.POINTER(C.c_float))
     slogs = np.zeros(len(logs), np.float32)
     res.log_ssim = slogs.ctypes.data_as(C.POINTER(C.c_float))
+    prender = np.zeros((h, w, 3), np.float32) if (want_render and o.get("packed_psnr")) else None
+    if prender is not None:
+        res.packed_render = prender.ctypes.data_as(C.POINTER(C.c_float))
     s = to_struct(o)
     _lib.call("sadh_fit", C.byref(s), _lib.ptr(tgt), _lib.ptr(msk), w, h, _lib.ptr(buf), cap, _lib.ptr(logs),
               len(logs), C.byref(res))
     out = _lib.fit_result_dict(res)
     out["grad_ms_trace"] = trace
     out["render"] = render
+    out["packed_render"] = prender
     out["sites"] = buf
     out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
     out["log_ssim"] = slogs[: min(res.n_logs, len(logs))].copy()
@@ -130,3 +134,34 @@ def render_sites(sites: np.ndarray, width: int, height: int, cand_passes: int = 
 def psnr(a: np.ndarray, b: np.ndarray) -> float:
     mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
     return 100.0 if mse <= 0 else 20.0 * np.log10(1.0 / np.sqrt(mse))
+
+
+def pack_inference(sites: np.ndarray, width: int, height: int):
+    """Quantised 16-byte inference sites of `sites` (sadh_pack_inference_sites, on
+    the current HIP device).  -> (packed uint32 [N][4], quant float32 [10]:
+    log_tau, radius, r, g, b as (min, scale) pairs)."""
+    import torch
+    s = torch.from_numpy(np.ascontiguousarray(sites, np.float32)).cuda()
+    n = s.shape[0]
+    packed = torch.zeros((max(n, 1), 4), dtype=torch.int32, device=s.device)
+    quant = torch.zeros(10, dtype=torch.float32, device=s.device)
+    _lib.call("sadh_pack_inference_sites", s.data_ptr(), n, width, height, packed.data_ptr(), quant.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return packed[:n].cpu().numpy().view(np.uint32), quant.cpu().numpy()
+
+
+def render_packed_sites(packed: np.ndarray, quant: np.ndarray, width: int, height: int, cand_passes: int = 16,
+                        use_jfa: bool = True, jfa_rounds: int = 1, cand_downscale: int = 1, radius_scale: float = 64.0,
+                        radius_probes: int = 0, inject_count: int = 4, hilbert_window: int = 32,
+                        hilbert_probes: int = 1) -> dict:
+    """render_sites on quantised inference sites (sadh_render_packed_sites;
+    renderPackedInference, backends/metal/sources/rendering/packed_renderer.swift:15-137)."""
+    p = np.ascontiguousarray(packed, np.uint32)
+    q = np.ascontiguousarray(quant, np.float32)
+    out = np.zeros((height, width, 3), np.float32)
+    cms, rms = C.c_float(), C.c_float()
+    _lib.call("sadh_render_packed_sites", _lib.ptr(p), _lib.ptr(q), p.shape[0], width, height, cand_passes,
+              int(use_jfa), jfa_rounds, cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window,
+              hilbert_probes, _lib.ptr(out), C.byref(cms), C.byref(rms))
+    return dict(image=out, cand_ms=cms.value, render_ms=rms.value)

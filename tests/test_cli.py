@@ -142,8 +142,11 @@ def test_cli_parse_defaults_and_flags():
         with pytest.raises(cli.UsageError):
             cli.parse_args(bad, cfg)
     assert cli.parse_args(["-h"], cfg)["mode"] == "help"
-    o = cli.parse_args(["x.png", "--ssim", "--ssim-weight", "0.25"], cfg)["opts"]
-    assert o["ssim_metric"] == 1 and o["ssim_weight"] == 0.25
+    o = cli.parse_args(["x.png", "--ssim", "--ssim-weight", "0.25", "--packed-psnr"], cfg)["opts"]
+    assert o["ssim_metric"] == 1 and o["ssim_weight"] == 0.25 and o["packed_psnr"] == 1
+    assert cli.parse_args(["--render", "s.txt", "--render-packed"], cfg)["render"]["packed"]
+    with pytest.raises(cli.UsageError):                 # only after --render (cli.swift:442-450)
+        cli.parse_args(["--render-packed", "--render", "s.txt"], cfg)
     assert cli.main(["--bogus"]) == 1
 
 
@@ -190,8 +193,16 @@ def test_cli_fit_and_render(tmp_path, capsys):
     assert np.array_equal(sites_io.load_image(str(rpng)), sites_io.to_8bit(job["image"]) / np.float32(255.0))
     psnr = float(log.split("Render PSNR: ")[1].split()[0])
     assert psnr > 20.0
+    # the 16-byte quantised sites rendered by the same job
+    ppng = tmp_path / "p.png"
+    rc = cli.main(["--render", str(txt), "--render-packed", "--out", str(ppng), "--render-target", str(img)])
+    assert rc == 0 and ppng.exists()
+    plog = capsys.readouterr().out
+    assert float(plog.split("Render PSNR: ")[1].split()[0]) > psnr - 1.0
     # a fit restarted from the written sites file
-    rc = cli.main([str(img), "--iters", "10", "--init-from-sites", str(txt), "--out-dir", str(tmp_path / "o2")])
-    assert rc == 0 and (tmp_path / "o2" / "scene_sites.txt").exists()
-    assert "init=sites" in capsys.readouterr().out
+    rc = cli.main([str(img), "--iters", "10", "--init-from-sites", str(txt), "--out-dir", str(tmp_path / "o2"),
+                   "--packed-psnr"])
+    assert rc == 0 and (tmp_path / "o2" / "scene_sites.txt").exists() and (tmp_path / "o2" / "scene_packed.png").exists()
+    log = capsys.readouterr().out
+    assert "init=sites" in log and "Packed inference PSNR" in log
     os.remove(str(rpng))
