@@ -140,7 +140,9 @@ int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite
 /* The trainer's per-iteration tail (sad_cuda.cu:2333-2389): the four
  * tau-diffusion passes (raw = grad_log_tau before the first pass) and Adam
  * with the gradient clear, the fourth pass and Adam fused into one kernel.
- * tau_scratch holds 2 * site_count floats.  Bit-identical to sadh_tau_diffuse
+ * tau_scratch holds 10 * site_count 32-bit words (two float arrays and the
+ * 8-id neighbour list of every site, built by the first pass and read by the
+ * others).  Bit-identical to sadh_tau_diffuse
  * x4 (grad_log_tau -> tmp -> grad_log_tau -> tmp -> grad_log_tau) followed by
  * sadh_adam_update_clear. */
 int sadh_tau_diffuse_adam(const uint32_t* cand0, const uint32_t* cand1, SadSite* sites, SadAdamState* adam,

@@ -54,6 +54,7 @@ struct FitRun {
     DevArena mem;
     float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr, *d_mse = nullptr;
     float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr, *removal = nullptr;
+    uint32_t* tau_nbr = nullptr;
     SadSite* d_sites = nullptr;
     SadAdamState* d_adam = nullptr;
     float* gacc = nullptr;
@@ -86,7 +87,8 @@ size_t fit_bytes(const SadhFitOptions& o, int width, int height, int cap, int pa
     const int ds = std::max(1, o.cand_downscale);
     const size_t cells = (size_t)((width + ds - 1) / ds) * ((height + ds - 1) / ds);
     size_t b = cells * 64 + np * (12 + 4 + 12);
-    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 64 + 16 + sizeof(SadSearchSite));
+    /* site, Adam state, gradient line, 4 tau buffers, tau neighbour list, search site */
+    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 64 + 16 + 32 + sizeof(SadSearchSite));
     if (o.densify_enabled) b += (size_t)cap * 32;
     if (pairs_count > 0) b += (size_t)pairs_count * 16 + 256 * 4 * (pairs_count / 1024 + 1) + 2 * 65536 * 4;
     if (use_hil) b += (size_t)cap * 8 + 16 * (size_t)(cap / 1024 + 1) * 1024 * 2;
@@ -153,6 +155,7 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.tau_raw, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp2, cap));
+    SAD_TRY(J.mem.alloc(&J.tau_nbr, (size_t)cap * 8));
     SAD_TRY(J.mem.alloc(&J.removal, cap));
     if (base.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(J.mem.alloc(&J.st[j], cap));
     if (J.pairs_count > 0) {
@@ -216,6 +219,7 @@ void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.tau_raw = J.tau_raw;
     d.tau_tmp = J.tau_tmp;
     d.tau_tmp2 = J.tau_tmp2;
+    d.tau_nbr = J.tau_nbr;
     d.removal = J.removal;
     d.h_order = J.h_order;
     d.h_pos = J.h_pos;

@@ -37,6 +37,7 @@ struct BatchJob {
     float* tau_raw;
     float* tau_tmp;
     float* tau_tmp2;
+    uint32_t* tau_nbr;         /* 8 ids per site: neighbour lists shared by the tau passes */
     float* removal;            /* prune iterations: removal deltas */
     const uint32_t* h_order;   /* null: no Hilbert probes */
     const uint32_t* h_pos;

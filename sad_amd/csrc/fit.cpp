@@ -186,6 +186,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&tau_raw, cap));
     SAD_TRY(mem.alloc(&tau_tmp, cap));
     SAD_TRY(mem.alloc(&tau_tmp2, cap));
+    uint32_t* tau_nbr;   /* per-site neighbour lists shared by the four tau passes */
+    SAD_TRY(mem.alloc(&tau_nbr, (size_t)cap * 8));
     SAD_TRY(mem.alloc(&removal, cap));
     if (o.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(mem.alloc(&st[j], cap));
     if (pairs_count > 0) {
@@ -362,7 +364,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
         SAD_TRY(launch_tau_adam_aos(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites, d_adam,
                                     lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                                    (uint32_t)(iter + 1), n, width, height, S));
+                                    (uint32_t)(iter + 1), n, width, height, S, tau_nbr));
 
         if (prune) {
             n_prune++;

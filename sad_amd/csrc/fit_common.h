@@ -26,9 +26,9 @@ int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const floa
 int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
                 float, uint32_t, uint32_t, int, int, bool, hipStream_t);
 int launch_clear(float* const*, int, uint32_t, hipStream_t);
-int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, float, int, int, int, SadSite*, SadAdamState*,
-                    float* const[10], float, float, float, float, float, float, float, float, float, uint32_t, uint32_t,
-                    int, int, hipStream_t);
+int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, uint32_t*, float, int, int, int, SadSite*,
+                    SadAdamState*, float* const[10], float, float, float, float, float, float, float, float, float,
+                    uint32_t, uint32_t, int, int, hipStream_t);
 int launch_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const float*, const float*, float*, float*,
                        uint32_t, float, int, int, int, hipStream_t);
 int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float* const[8], uint32_t, const float*,
@@ -107,9 +107,10 @@ int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int heigh
                           SadInferenceQuant* quant, hipStream_t stream);
 int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
                             int height, SadSite* out, hipStream_t stream);
+/* nbr_lists (nullable): 8 uint32 per site of scratch for the neighbour lists the passes share */
 int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
                         SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
-                        uint32_t, int, int, hipStream_t);
+                        uint32_t, int, int, hipStream_t, uint32_t* nbr_lists = nullptr);
 
 /* batched fit (one launch over every job) */
 int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);

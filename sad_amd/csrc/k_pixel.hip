@@ -20,9 +20,9 @@
  *   4. a scan of the per-slot counts turns ranks into positions of a per-slot
  *      entry list (pixel, candidate rank), and cuts every slot's list into
  *      chunks of CHUNK entries.
- *   5. chunk-parallel accumulation: 8 lanes per chunk differentiate its
- *      (pixel, candidate) pairs and sum in registers; an 8-lane DPP reduction
- *      leaves one partial per chunk in LDS; the partials of each slot are then
+ *   5. chunk-parallel accumulation: CLANES (4) lanes per chunk differentiate
+ *      its (pixel, candidate) pairs and sum in registers; a DPP reduction over
+ *      the lane group leaves one partial per chunk in LDS; the partials of each slot are then
  *      summed (one thread per site and parameter) into one global float atomic
  *      per (site, parameter) and tile.  Chunking balances the work: the entry
  *      counts of a tile's sites range from a few to all 256 pixels, and giving
@@ -61,7 +61,20 @@ constexpr uint16_t ENTRY_FIRST = 0x8000u;    /* entry flag: first rank of its si
 #ifndef SAD_CHUNK
 #define SAD_CHUNK 24
 #endif
-constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (3 per lane of an 8-lane group) */
+constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (6 per lane of a 4-lane group) */
+#ifndef SAD_CHUNK_LANES
+#define SAD_CHUNK_LANES 4
+#endif
+constexpr uint32_t CLANES = SAD_CHUNK_LANES;   /* lanes per chunk: 4 or 8 */
+static_assert(CLANES == 4u || CLANES == 8u, "chunks take 4 or 8 lanes");
+
+/* sum over the CLANES-lane group of a lane; every lane of the group gets it */
+__device__ __forceinline__ float chunk_sum(float v)
+{
+    if constexpr (CLANES == 8u) return group8_sum(v);
+    v = v + dpp<0xB1>(v);
+    return v + dpp<0x4E>(v);
+}
 constexpr uint8_t SLOT8_NONE = 0xFFu;        /* per-pixel table: candidate not accumulated via a slot */
 
 /* values accumulated per site: 10 gradients (+ removal delta), or 8 statistics */
@@ -461,11 +474,11 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     }
     __syncthreads();
 
-    /* stage 5: 8 lanes per chunk of CHUNK entries; one partial per chunk */
+    /* stage 5: CLANES lanes per chunk of CHUNK entries; one partial per chunk */
     const uint32_t n_chunks = L.n_chunks;
-    const uint32_t part = tid & 7u;
-    for (uint32_t base = 0; base < n_chunks; base += 32u) {
-        const uint32_t q = base + (tid >> 3);
+    const uint32_t part = tid & (CLANES - 1u);
+    for (uint32_t base = 0; base < n_chunks; base += 256u / CLANES) {
+        const uint32_t q = base + tid / CLANES;
         const bool live = q < n_chunks;
         const uint32_t s = live ? (uint32_t)L.chunk_slot[q] : 0u;
         const uint32_t sbeg = L.start[s];
@@ -478,7 +491,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         float acc[NV];
 #pragma unroll
         for (int j = 0; j < NV; ++j) acc[j] = 0.0f;
-        for (uint32_t e = beg + part; e < end; e += 8u) {
+        for (uint32_t e = beg + part; e < end; e += CLANES) {
             const uint32_t ent = L.list[e];
             const uint32_t p = (ent & 0x7FFFu) >> 3, k = ent & 7u;
             const float ex = (float)(tx0 + (int)(p & 15u)), ey = (float)(ty0 + (int)(p >> 4));
@@ -511,13 +524,13 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
                 for (int j = 0; j < 10; ++j) acc[j] += c[j];
             }
         }
-        /* every lane of the group ends with the same total; lane `part` stores values part and part + 8 */
+        /* every lane of the group ends with the same total; lane `part` stores the values j = part mod CLANES */
 #pragma unroll
-        for (int j = 0; j < NV; ++j) acc[j] = group8_sum(acc[j]);
+        for (int j = 0; j < NV; ++j) acc[j] = chunk_sum(acc[j]);
         if (live) {
 #pragma unroll
             for (int j = 0; j < NV; ++j)
-                if ((uint32_t)(j & 7) == part) L.part[q][j] = acc[j];
+                if (((uint32_t)j & (CLANES - 1u)) == part) L.part[q][j] = acc[j];
         }
     }
     __syncthreads();

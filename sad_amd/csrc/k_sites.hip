@@ -156,6 +156,109 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
     gout[i] = (raw + lambda * sum) / diag;
 }
 
+/* ------------------------------------------------ tau passes from lists --
+ * The four passes of one iteration read the same neighbour sets: positions,
+ * the candidate field and activity do not change between them.  The first pass
+ * writes each site's set as 8 ids in candidate order (SAD_EMPTY for ids that do
+ * not count, NBR_COPY in slot 0 when the pass copies gin through), and the
+ * later passes read that list instead of chasing position -> home cell ->
+ * candidates -> neighbour activity again.  Sums run in the same order as in
+ * tau_site, so the results are bit-identical. */
+#define NBR_COPY 0xFFFFFFFEu
+
+__device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                              const SadSite* __restrict__ sites, const float* __restrict__ graw,
+                                              const float* __restrict__ gin, float* __restrict__ gout,
+                                              float* __restrict__ raw_copy, uint32_t n, float lambda, int width,
+                                              int height, float inv_ds, uint32_t in_stride, uint4* __restrict__ nbr)
+{
+    const float raw = graw[i * in_stride];
+    if (raw_copy) raw_copy[i] = raw;
+    const float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
+    uint32_t keep[8];
+    if (lambda <= 0.0f || pos.x < 0.0f) {
+        gout[i] = gin[i * in_stride];
+        nbr[2u * i] = make_uint4(NBR_COPY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        nbr[2u * i + 1u] = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        return;
+    }
+    const int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
+    const int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
+    const uint4 a = c0[py * width + px], b = c1[py * width + px];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float sum = 0.0f;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t nb = ids[k];
+        bool ok = nb < n && nb != i;
+#pragma unroll
+        for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
+        ok = ok && sites[nb].pos_x >= 0.0f;
+        keep[k] = ok ? nb : SAD_EMPTY;
+        if (ok) {
+            sum += gin[nb * in_stride];
+            cnt += 1u;
+        }
+    }
+    nbr[2u * i] = make_uint4(keep[0], keep[1], keep[2], keep[3]);
+    nbr[2u * i + 1u] = make_uint4(keep[4], keep[5], keep[6], keep[7]);
+    gout[i] = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
+}
+
+/* one later pass from the list: the value tau_site computes */
+__device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const float* __restrict__ graw,
+                                               const float* __restrict__ gin, float lambda)
+{
+    const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
+    if (a.x == NBR_COPY) return gin[i];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float sum = 0.0f;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (ids[k] != SAD_EMPTY) {
+            sum += gin[ids[k]];
+            cnt += 1u;
+        }
+    }
+    return (graw[i] + lambda * sum) / (1.0f + lambda * (float)cnt);
+}
+
+__global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                        const SadSite* __restrict__ sites, const float* __restrict__ g2,
+                                                        float* __restrict__ gout, float* __restrict__ raw_copy,
+                                                        uint32_t n, float lambda, int width, int height, float inv_ds,
+                                                        uint4* __restrict__ nbr, uint32_t in_stride)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) tau_site_list(i, c0, c1, sites, g2, g2, gout, raw_copy, n, lambda, width, height, inv_ds, in_stride,
+                             nbr);
+}
+
+__global__ __launch_bounds__(256) void k_tau_list_pass(const uint4* __restrict__ nbr, const float* __restrict__ graw,
+                                                       const float* __restrict__ gin, float* __restrict__ gout,
+                                                       uint32_t n, float lambda)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) gout[i] = tau_from_list(i, nbr, graw, gin, lambda);
+}
+
+__global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restrict__ nbr,
+                                                            const float* __restrict__ graw,
+                                                            const float* __restrict__ gin, float lambda, AdamArgs A)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= A.n) return;
+    const float v = tau_from_list(i, nbr, graw, gin, lambda);
+    float g[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+    adam_site(A, i, g);
+}
+
 /* first pass reading the log_tau column of AoS [N][16] gradient lines */
 __global__ __launch_bounds__(256) void k_tau_pass_strided(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
                                                           const SadSite* __restrict__ sites,
@@ -249,13 +352,14 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
     const uint4* c0 = reinterpret_cast<const uint4*>(J.cur ? J.fb0 : J.fa0);
     const uint4* c1 = reinterpret_cast<const uint4*>(J.cur ? J.fb1 : J.fa1);
     const float inv_ds = 1.0f / fmaxf((float)J.ds, 1.0f);
+    uint4* nbr = reinterpret_cast<uint4*>(J.tau_nbr);
     if (pass == 0u) {
-        tau_site(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, js.lambda, J.cw, J.ch, inv_ds,
-                 16u);
+        tau_site_list(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, js.lambda, J.cw, J.ch,
+                      inv_ds, 16u, nbr);
     } else if (pass == 1u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp, J.tau_tmp2, nullptr, J.n, js.lambda, J.cw, J.ch, inv_ds);
+        J.tau_tmp2[i] = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda);
     } else if (pass == 2u) {
-        tau_site(i, c0, c1, J.sites, J.tau_raw, J.tau_tmp2, J.tau_tmp, nullptr, J.n, js.lambda, J.cw, J.ch, inv_ds);
+        J.tau_tmp[i] = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp2, js.lambda);
     } else {
         AdamArgs A;
         A.sites = J.sites; A.st = J.adam;
@@ -265,7 +369,13 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
         A.b1c = js.b1c; A.b2c = js.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
-        tau_last_adam_site(i, c0, c1, J.tau_raw, J.tau_tmp, js.lambda, J.cw, J.ch, inv_ds, A);
+        const float v = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda);
+        float g[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+        adam_site(A, i, g);
     }
 }
 
@@ -741,24 +851,25 @@ int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* si
 
 /* The trainer's tail: tau passes 1-3 (g2 -> tmp -> g2 -> tmp, raw saved by the
  * first) and pass 4 fused with Adam + clear. */
-int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* tmp, float lambda, int cand_width,
-                    int cand_height, int ds, SadSite* sites, SadAdamState* st, float* const g[10], float lr_pos,
-                    float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso, float beta1,
-                    float beta2, float eps, uint32_t t, uint32_t n, int width, int height, hipStream_t stream)
+/* SoA gradients; nbr: 8 uint32 per site of neighbour-list scratch */
+int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* tmp, uint32_t* nbr_lists, float lambda,
+                    int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st, float* const g[10],
+                    float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
+                    float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
+                    hipStream_t stream)
 {
     if (n == 0) return SADH_OK;
-    int rc = launch_tau_diffuse(c0, c1, sites, g[2], g[2], tmp, raw, n, lambda, cand_width, cand_height, ds, stream);
-    if (!rc) rc = launch_tau_diffuse(c0, c1, sites, raw, tmp, g[2], nullptr, n, lambda, cand_width, cand_height, ds,
-                                     stream);
-    if (!rc) rc = launch_tau_diffuse(c0, c1, sites, raw, g[2], tmp, nullptr, n, lambda, cand_width, cand_height, ds,
-                                     stream);
-    if (rc) return rc;
+    const dim3 grid(div_up(n, 256u));
+    const uint4* nbr = reinterpret_cast<const uint4*>(nbr_lists);
+    hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
+                       reinterpret_cast<const uint4*>(c1), sites, g[2], tmp, raw, n, lambda, cand_width, cand_height,
+                       1.0f / fmaxf((float)ds, 1.0f), reinterpret_cast<uint4*>(nbr_lists), 1u);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, raw, tmp, g[2], n, lambda);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, raw, g[2], tmp, n, lambda);
     AdamArgs A;
     fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
               height);
-    hipLaunchKernelGGL(k_tau_last_adam, dim3(div_up(n, 256u)), dim3(256), 0, stream,
-                       reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), raw, tmp, lambda,
-                       cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), A);
+    hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, nbr, raw, tmp, lambda, A);
     return launch_status("tau_diffuse_adam");
 }
 
@@ -797,25 +908,34 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
                         float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                         float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                         float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                        hipStream_t stream)
+                        hipStream_t stream, uint32_t* nbr_lists)
 {
     if (n == 0) return SADH_OK;
     const float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
     const dim3 grid(div_up(n, 256u));
     const uint4* u0 = reinterpret_cast<const uint4*>(c0);
     const uint4* u1 = reinterpret_cast<const uint4*>(c1);
-    hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
-                       cand_width, cand_height, inv_ds);
-    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp, tmp2, (float*)nullptr, n,
-                       lambda, cand_width, cand_height, inv_ds);
-    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp2, tmp, (float*)nullptr, n,
-                       lambda, cand_width, cand_height, inv_ds);
     float* g[10];
     for (int j = 0; j < 10; ++j) g[j] = gacc + j;
     AdamArgs A;
     fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
               height);
     A.gstride = 16u;
+    if (nbr_lists) {
+        uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
+        hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
+                           cand_width, cand_height, inv_ds, nbr, 16u);
+        hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, tmp2, n, lambda);
+        hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp2, tmp, n, lambda);
+        hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, lambda, A);
+        return launch_status("tau_diffuse_adam(aos, lists)");
+    }
+    hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
+                       cand_width, cand_height, inv_ds);
+    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp, tmp2, (float*)nullptr, n,
+                       lambda, cand_width, cand_height, inv_ds);
+    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp2, tmp, (float*)nullptr, n,
+                       lambda, cand_width, cand_height, inv_ds);
     hipLaunchKernelGGL(k_tau_last_adam, grid, dim3(256), 0, stream, u0, u1, raw, tmp, lambda, cand_width,
                        cand_height, inv_ds, A);
     return launch_status("tau_diffuse_adam(aos)");
@@ -1037,10 +1157,11 @@ int sadh_tau_diffuse_adam(const uint32_t* cand0, const uint32_t* cand1, SadSite*
     for (int j = 0; j < 10; ++j) SADH_REQUIRE(g[j], "tau_diffuse_adam: null gradient array");
     SADH_REQUIRE(cand0 && cand1 && sites && adam && tau_scratch, "tau_diffuse_adam: null pointer");
     SADH_REQUIRE(cand_width > 0 && cand_height > 0 && width > 0 && height > 0, "tau_diffuse_adam: bad sizes");
-    return launch_tau_adam(cand0, cand1, tau_scratch, tau_scratch + site_count, lambda, cand_width, cand_height,
-                           cand_downscale, sites, adam, g,
-                           lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, site_count,
-                           width, height, to_hip(stream));
+    /* tau_scratch: raw [n], tmp [n], neighbour lists [n][8] */
+    return launch_tau_adam(cand0, cand1, tau_scratch, tau_scratch + site_count,
+                           reinterpret_cast<uint32_t*>(tau_scratch + 2 * (size_t)site_count), lambda, cand_width,
+                           cand_height, cand_downscale, sites, adam, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir,
+                           lr_aniso, beta1, beta2, eps, t, site_count, width, height, to_hip(stream));
 }
 
 int sadh_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite* sites, const float* grad_raw,

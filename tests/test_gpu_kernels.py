@@ -330,7 +330,7 @@ def test_tau_diffuse_adam_fused(gpu, orc, w, h, n, lam):
     # fused
     s_b, st_b = gpu.put(sites), gpu.put(state)
     gb = [gpu.put(grads[j]) for j in range(10)]
-    gpu.call("sadh_tau_diffuse_adam", g0, g1, s_b, st_b, *gb, gpu.zeros((2 * n,)), lam, w, h, 1, *lrs, 0.9, 0.999, 1e-8,
+    gpu.call("sadh_tau_diffuse_adam", g0, g1, s_b, st_b, *gb, gpu.zeros((10 * n,)), lam, w, h, 1, *lrs, 0.9, 0.999, 1e-8,
              7, n, w, h)
     assert np.array_equal(gpu.get(s_b), gpu.get(s_a))
     assert np.array_equal(gpu.get(st_b), gpu.get(st_a))
