@@ -58,6 +58,20 @@ def pmc_traffic(workload: str):
     return None
 
 
+def pmc_valu_busy(workload: str):
+    """VALU busy fraction of the gradient kernel (SQ_ACTIVE_INST_VALU x 4 over
+    SIMDs x busy cycles) from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"k_pixel_grad_{workload}_valu.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("valu_busy:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def kernel_workload(args) -> int:
     """--workload render4096: per-kernel timings at 4096x4096 / 1,048,576 sites
     (sad_amd.kernel_bench), and the CPU oracle on the same inputs."""
@@ -286,12 +300,14 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload, "width": w, "height": h, "init_sites": int(sites.shape[0]),
                    "iterations": iters, "final_sites": last["actual_sites"], "active_sites": last["active_count"],
-                   "final_psnr_db": round(last["final_psnr"], 3), "wall_s": round(wall, 3)},
+                   "final_psnr_db": round(last["final_psnr"], 3),
+                   "final_psnr_db_steps": [round(r["final_psnr"], 3) for r in runs], "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload),
                      "algorithmic_bytes_per_launch": int(gbytes / max(1, glaunch)),
                      "kernel": "k_pixel<MODE_GRAD> (compute_gradients_tiled)",
-                     "avg_launch_us": round(1e3 * gms / max(1, glaunch), 3)},
+                     "avg_launch_us": round(1e3 * gms / max(1, glaunch), 3),
+                     "valu_busy": pmc_valu_busy(args.workload)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
