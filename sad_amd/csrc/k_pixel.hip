@@ -103,7 +103,9 @@ struct TileLds : TileSites<H> {
     uint16_t cstart[H];        /* first chunk of each slot */
     uint32_t n_chunks;
     uint32_t n_ovf;            /* pairs whose id found no hash slot */
-    uint8_t slot[256][8];      /* per pixel: slot of each candidate rank (SLOT8_NONE if unused) */
+    /* per pixel: slot of each candidate rank (SLOT8_NONE if unused); read only to sum a site's weight over
+     * its ranks in a pixel (removal deltas, statistics) */
+    uint8_t slot[(REMOVAL || NV == 8) ? 256 : 1][8];
     float w[256][8];           /* per pixel: softmax weights */
     float4 px[256];            /* per pixel: pred.rgb, err (STATS) */
     float4 pg[256];            /* per pixel: dL/dpred.rgb */
@@ -360,7 +362,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         const bool use = contrib && valid[k];
         sl[k] = (use && s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
         rank[k] = (sl[k] != SLOT8_NONE) ? atomicAdd(&L.count[s], 1u) : 0u;
-        L.slot[tid][k] = sl[k];
+        if constexpr (MODE == MODE_STATS || REMOVAL) L.slot[tid][k] = sl[k];
         if (use && s == SLOT_OVERFLOW) L.list[atomicAdd(&L.n_ovf, 1u)] = (uint16_t)((tid << 3) | (uint32_t)k);
     }
     __syncthreads();

@@ -206,23 +206,71 @@ __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restric
     gout[i] = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
-/* one later pass from the list: the value tau_site computes */
+/* the list alone (tau_site_list without the pass): written before the
+ * gradients exist, off their critical path */
+__device__ __forceinline__ void tau_list_only(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                              const SadSite* __restrict__ sites, uint32_t n, float lambda, int width,
+                                              int height, float inv_ds, uint4* __restrict__ nbr)
+{
+    const float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
+    if (lambda <= 0.0f || pos.x < 0.0f) {
+        nbr[2u * i] = make_uint4(NBR_COPY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        nbr[2u * i + 1u] = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        return;
+    }
+    const int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
+    const int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
+    const uint4 a = c0[py * width + px], b = c1[py * width + px];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t keep[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t nb = ids[k];
+        bool ok = nb < n && nb != i;
+#pragma unroll
+        for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
+        keep[k] = (ok && sites[nb].pos_x >= 0.0f) ? nb : SAD_EMPTY;
+    }
+    nbr[2u * i] = make_uint4(keep[0], keep[1], keep[2], keep[3]);
+    nbr[2u * i + 1u] = make_uint4(keep[4], keep[5], keep[6], keep[7]);
+}
+
+/* one pass from the list: the value tau_site computes (graw and gin hold site i at [i * stride]) */
 __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const float* __restrict__ graw,
-                                               const float* __restrict__ gin, float lambda)
+                                               const float* __restrict__ gin, float lambda, uint32_t stride = 1u)
 {
     const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
-    if (a.x == NBR_COPY) return gin[i];
+    if (a.x == NBR_COPY) return gin[i * stride];
     const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     float sum = 0.0f;
     uint32_t cnt = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (ids[k] != SAD_EMPTY) {
-            sum += gin[ids[k]];
+            sum += gin[ids[k] * stride];
             cnt += 1u;
         }
     }
-    return (graw[i] + lambda * sum) / (1.0f + lambda * (float)cnt);
+    return (graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
+}
+
+__global__ __launch_bounds__(256) void k_tau_lists(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                   const SadSite* __restrict__ sites, uint32_t n, float lambda,
+                                                   int width, int height, float inv_ds, uint4* __restrict__ nbr)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) tau_list_only(i, c0, c1, sites, n, lambda, width, height, inv_ds, nbr);
+}
+
+/* first pass from built lists, reading the log_tau column of the gradient lines; saves raw */
+__global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __restrict__ nbr,
+                                                              const float* __restrict__ g2, float* __restrict__ gout,
+                                                              float* __restrict__ raw_copy, uint32_t n, float lambda)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    raw_copy[i] = g2[i * 16u];
+    gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
 }
 
 __global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
@@ -908,7 +956,7 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
                         float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                         float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                         float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                        hipStream_t stream, uint32_t* nbr_lists)
+                        hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
 {
     if (n == 0) return SADH_OK;
     const float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
@@ -923,8 +971,12 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
     A.gstride = 16u;
     if (nbr_lists) {
         uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
-        hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
-                           cand_width, cand_height, inv_ds, nbr, 16u);
+        if (lists_built)
+            hipLaunchKernelGGL(k_tau_list_first_built, grid, dim3(256), 0, stream, (const uint4*)nbr, gacc + 2, tmp,
+                               raw, n, lambda);
+        else
+            hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n,
+                               lambda, cand_width, cand_height, inv_ds, nbr, 16u);
         hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, tmp2, n, lambda);
         hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp2, tmp, n, lambda);
         hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, lambda, A);
@@ -939,6 +991,16 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
     hipLaunchKernelGGL(k_tau_last_adam, grid, dim3(256), 0, stream, u0, u1, raw, tmp, lambda, cand_width,
                        cand_height, inv_ds, A);
     return launch_status("tau_diffuse_adam(aos)");
+}
+
+int launch_tau_lists(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, uint32_t n, float lambda, int cw,
+                     int ch, int ds, uint32_t* nbr_lists, hipStream_t stream)
+{
+    if (n == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_tau_lists, dim3(div_up(n, 256u)), dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
+                       reinterpret_cast<const uint4*>(c1), sites, n, lambda, cw, ch, 1.0f / fmaxf((float)ds, 1.0f),
+                       reinterpret_cast<uint4*>(nbr_lists));
+    return launch_status("tau_lists");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
