@@ -52,13 +52,15 @@ struct FitRun {
     size_t cells = 0;
     float mask_sum = 0.0f, iss = 0.0f, delta_norm = 0.0f;
     DevArena mem;
-    float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr, *d_mse = nullptr;
-    float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr, *removal = nullptr;
+    float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr;
+    double* d_mse = nullptr;
+    float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr;
+    double* removal = nullptr;
     uint32_t* tau_nbr = nullptr;
     SadSite* d_sites = nullptr;
     SadAdamState* d_adam = nullptr;
-    float* gacc = nullptr;
-    float* st[8] = {};
+    double* gacc = nullptr;
+    double* st[8] = {};
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count = nullptr;
     uint32_t *fa0 = nullptr, *fa1 = nullptr, *fb0 = nullptr, *fb1 = nullptr;
@@ -69,7 +71,8 @@ struct FitRun {
     int actual = 0, active_est = 0;
     uint32_t n = 0, cur = 0, hil_count = 0;
     bool hil_ready = false;
-    float best = 0.0f, final_psnr = 0.0f, sse = 0.0f;
+    float best = 0.0f, final_psnr = 0.0f;
+    double sse = 0.0;
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
     double submit_s = 0.0, admit_s = 0.0;
@@ -87,9 +90,9 @@ size_t fit_bytes(const SadhFitOptions& o, int width, int height, int cap, int pa
     const int ds = std::max(1, o.cand_downscale);
     const size_t cells = (size_t)((width + ds - 1) / ds) * ((height + ds - 1) / ds);
     size_t b = cells * 64 + np * (12 + 4 + 12);
-    /* site, Adam state, gradient line, 4 tau buffers, tau neighbour list, search site */
-    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 64 + 16 + 32 + sizeof(SadSearchSite));
-    if (o.densify_enabled) b += (size_t)cap * 32;
+    /* site, Adam state, double gradient line, 3 tau buffers, double removal delta, tau neighbour list, search site */
+    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 128 + 12 + 8 + 32 + sizeof(SadSearchSite));
+    if (o.densify_enabled) b += (size_t)cap * 64;
     if (pairs_count > 0) b += (size_t)pairs_count * 16 + 256 * 4 * (pairs_count / 1024 + 1) + 2 * 65536 * 4;
     if (use_hil) b += (size_t)cap * 8 + 16 * (size_t)(cap / 1024 + 1) * 1024 * 2;
     return b;
@@ -182,8 +185,8 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     HIP_TRY(hipMemcpyAsync(J.d_mask, mask.data(), sizeof(float) * J.np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(J.d_sites, hs.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(J.d_adam, 0, sizeof(SadAdamState) * cap, S));
-    HIP_TRY(hipMemsetAsync(J.gacc, 0, sizeof(float) * 16 * cap, S));
-    HIP_TRY(hipMemsetAsync(J.removal, 0, sizeof(float) * cap, S));
+    HIP_TRY(hipMemsetAsync(J.gacc, 0, sizeof(double) * 16 * cap, S));
+    HIP_TRY(hipMemsetAsync(J.removal, 0, sizeof(double) * cap, S));
     J.actual = io.n_sites;
     J.n = (uint32_t)J.actual;
     J.active_est = 0;
@@ -434,8 +437,8 @@ struct Engine {
             if (!J.dens || J.pairs_count <= 0) continue;
             J.n_dens++;
             const int desired = std::max(0, (int)((float)J.active_est * J.d_pct));
-            SAD_TRY(launch_clear(J.st, 8, J.n, S));
-            SAD_TRY(launch_site_stats(J.c0(), J.c1(), J.d_target, J.d_mask, J.d_sites, J.iss, J.n, J.st, J.width,
+            SAD_TRY(launch_clear64(J.st, 8, J.n, S));
+            SAD_TRY(launch_site_stats64(J.c0(), J.c1(), J.d_target, J.d_mask, J.d_sites, J.iss, J.n, J.st, J.width,
                                       J.height, J.cw, J.ch, S));
             SAD_TRY(launch_densify_pairs(J.d_sites, J.st[0], J.st[1], J.pairs, J.n, 1.0f, o.densify_score_alpha,
                                          (uint32_t)J.pairs_count, S));
@@ -458,7 +461,7 @@ struct Engine {
             SAD_TRY(launch_update_jobs(d_jobs, upd_idx.data(), upd_n.data(), upd_flip.data(), (uint32_t)upd_idx.size(),
                                        S));
         if (any_prune)
-            for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear(&fits[j]->removal, 1, fits[j]->n, S));
+            for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear64(&fits[j]->removal, 1, fits[j]->n, S));
 
         if (o.profile_gradients) {
             hipEvent_t a, b;
@@ -509,16 +512,16 @@ struct Engine {
                 if (!J.log) continue;
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
                                       J.ch, S));
-                SAD_TRY(launch_sse(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
+                SAD_TRY(launch_sse64(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
                 SAD_TRY(launch_count_active(J.d_sites, J.d_count, J.n, S));
-                HIP_TRY(hipMemcpyAsync(&J.sse, J.d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
+                HIP_TRY(hipMemcpyAsync(&J.sse, J.d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
                 HIP_TRY(hipMemcpyAsync(&J.active_count, J.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             }
             HIP_TRY(hipStreamSynchronize(S));
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.log) continue;
-                float mse = J.sse / (std::max(1.0f, J.mask_sum) * 3.0f);
+                float mse = (float)J.sse / (std::max(1.0f, J.mask_sum) * 3.0f);
                 float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
                 if (J.io.log_psnr && J.n_logs < J.io.log_cap) J.io.log_psnr[J.n_logs] = psnr;
                 J.n_logs++;

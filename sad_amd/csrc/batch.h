@@ -33,12 +33,12 @@ struct BatchJob {
     SadSite* sites;
     SadAdamState* adam;
     SadSearchSite* search;
-    float* gacc;               /* gradients: AoS [cap][16] lines, values 0-9 in Site field order */
+    double* gacc;              /* gradients: AoS [cap][16] double lines, values 0-9 in Site field order */
     float* tau_raw;
     float* tau_tmp;
     float* tau_tmp2;
     uint32_t* tau_nbr;         /* 8 ids per site: neighbour lists shared by the tau passes */
-    float* removal;            /* prune iterations: removal deltas */
+    double* removal;           /* prune iterations: removal deltas */
     const uint32_t* h_order;   /* null: no Hilbert probes */
     const uint32_t* h_pos;
     uint32_t n;                /* current site count */

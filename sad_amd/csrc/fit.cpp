@@ -164,11 +164,17 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     }
 
     DevArena mem;
-    float *d_target, *d_mask, *d_render, *d_mse, *tau_raw, *tau_tmp, *tau_tmp2, *removal;
+    float *d_target, *d_mask, *d_render, *tau_raw, *tau_tmp, *tau_tmp2;
+    double* d_mse;
     SadSite* d_sites;
     SadAdamState* d_adam;
-    float* gacc;   /* gradients as AoS [cap][16] lines: one 64-byte atomic target per site */
-    float* st[8] = {};
+    /* Cross-tile sums in double: gradients as AoS [cap][16] lines (one 128-byte atomic target per site),
+     * removal deltas and site statistics.  Each tile adds one float partial per site and value; summed in
+     * double and rounded once where they are read, the values do not depend on the order in which tiles
+     * finish, so a fit repeats exactly. */
+    double* gacc;
+    double* removal;
+    double* st[8] = {};
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count;
     uint32_t *cA0, *cA1, *cB0, *cB1;
@@ -177,7 +183,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&d_mask, (size_t)np));
     SAD_TRY(mem.alloc(&d_render, (size_t)np * 3));
     SAD_TRY(mem.alloc(&d_mse, 1));
-    float* d_ssim;
+    double* d_ssim;
     SAD_TRY(mem.alloc(&d_ssim, 1));
     SAD_TRY(mem.alloc(&d_count, 1));
     SAD_TRY(mem.alloc(&d_sites, cap));
@@ -226,8 +232,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     HIP_TRY(hipMemcpyAsync(d_mask, mask.data(), sizeof(float) * np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(d_sites, host_sites.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(d_adam, 0, sizeof(SadAdamState) * cap, S));
-    HIP_TRY(hipMemsetAsync(gacc, 0, sizeof(float) * 16 * cap, S));
-    HIP_TRY(hipMemsetAsync(removal, 0, sizeof(float) * cap, S));
+    HIP_TRY(hipMemsetAsync(gacc, 0, sizeof(double) * 16 * cap, S));
+    HIP_TRY(hipMemsetAsync(removal, 0, sizeof(double) * cap, S));
 
     int actual = n0;
     uint32_t n = (uint32_t)actual;
@@ -318,8 +324,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         if (densify && o.densify_enabled && needs_pairs) {
             n_dens++;
-            SAD_TRY(launch_clear(st, 8, n, S));
-            SAD_TRY(launch_site_stats(cA0, cA1, d_target, d_mask, d_sites, iss, n, st, width, height, cw, ch, S));
+            SAD_TRY(launch_clear64(st, 8, n, S));
+            SAD_TRY(launch_site_stats64(cA0, cA1, d_target, d_mask, d_sites, iss, n, st, width, height, cw, ch, S));
             SAD_TRY(launch_densify_pairs(d_sites, st[0], st[1], pairs, n, 1.0f, o.densify_score_alpha,
                                          (uint32_t)pairs_count, S));
             /* pairs at index >= n hold the (0xFFFFFFFF, 0xFFFFFFFF) sentinel: the stable sort of the
@@ -354,7 +360,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         const bool prune = p_pct > 0.0f && iter >= eff_ps && iter < eff_pe && iter % std::max(1, o.prune_freq) == 0 &&
                            needs_pairs && needs_prune;
-        if (prune) SAD_TRY(launch_clear(&removal, 1, n, S));
+        if (prune) SAD_TRY(launch_clear64(&removal, 1, n, S));
 
         if (o.profile_gradients) {
             hipEvent_t a, b;
@@ -366,8 +372,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         /* the SSIM loss term reads a render of the current state (sad_cuda.cu:2307-2318) */
         if (o.ssim_weight > 0.0f) SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
-        SAD_TRY(launch_gradients_aos(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width, height,
-                                     cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f));
+        SAD_TRY(launch_gradients_lines64(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width,
+                                         height, cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             gbytes.push_back(48.0 * np + 120.0 * n + (prune ? 8.0 * n : 0.0));
@@ -379,9 +385,9 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         if (!densify) HIP_TRY(hipStreamWaitEvent(S, sp.lists_done, 0));
         /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
-        SAD_TRY(launch_tau_adam_aos(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites, d_adam,
-                                    lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                                    (uint32_t)(iter + 1), n, width, height, S, tau_nbr, true));
+        SAD_TRY(launch_tau_adam_lines64(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites,
+                                        d_adam, lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
+                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr, true));
 
         if (prune) {
             n_prune++;
@@ -398,19 +404,19 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
 
         if (iter % std::max(1, o.log_freq) == 0 || iter == o.iterations - 1) {
             SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
-            SAD_TRY(launch_sse(d_render, d_target, d_mask, d_mse, width, height, S));
-            if (o.ssim_metric) SAD_TRY(launch_ssim(d_render, d_target, d_mask, d_ssim, width, height, S));
+            SAD_TRY(launch_sse64(d_render, d_target, d_mask, d_mse, width, height, S));
+            if (o.ssim_metric) SAD_TRY(launch_ssim64(d_render, d_target, d_mask, d_ssim, width, height, S));
             SAD_TRY(launch_count_active(d_sites, d_count, n, S));
-            float sse = 0.0f, ssim_sum = 0.0f;
-            HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
-            if (o.ssim_metric) HIP_TRY(hipMemcpyAsync(&ssim_sum, d_ssim, sizeof(float), hipMemcpyDeviceToHost, S));
+            double sse = 0.0, ssim_sum = 0.0;
+            HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
+            if (o.ssim_metric) HIP_TRY(hipMemcpyAsync(&ssim_sum, d_ssim, sizeof(double), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipMemcpyAsync(&active_count, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipStreamSynchronize(S));
-            float mse = sse / (std::max(1.0f, mask_sum) * 3.0f);
+            float mse = (float)sse / (std::max(1.0f, mask_sum) * 3.0f);
             float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
             if (log_psnr && n_logs < log_cap) log_psnr[n_logs] = psnr;
             if (o.ssim_metric) {
-                const float ssim = ssim_sum / std::max(1.0f, mask_sum);   /* sad_cuda.cu:2443-2448 */
+                const float ssim = (float)ssim_sum / std::max(1.0f, mask_sum);   /* sad_cuda.cu:2443-2448 */
                 if (res && res->log_ssim && n_logs < log_cap) res->log_ssim[n_logs] = ssim;
                 final_ssim = ssim;
                 best_ssim = std::max(best_ssim, ssim);
@@ -447,13 +453,13 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         SAD_TRY(launch_pack_inference(d_sites, n, width, height, d_packed, d_quant, S));
         SAD_TRY(launch_unpack_inference(d_packed, d_quant, n, width, height, d_dec, S));
         SAD_TRY(launch_render(cA0, cA1, d_render, d_dec, iss, n, width, height, cw, ch, S));
-        SAD_TRY(launch_sse(d_render, d_target, d_mask, d_mse, width, height, S));
-        float sse = 0.0f;
-        HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(float), hipMemcpyDeviceToHost, S));
+        SAD_TRY(launch_sse64(d_render, d_target, d_mask, d_mse, width, height, S));
+        double sse = 0.0;
+        HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
         if (res && res->packed_render)
             HIP_TRY(hipMemcpyAsync(res->packed_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
         HIP_TRY(hipStreamSynchronize(S));
-        const float mse = sse / (std::max(1.0f, mask_sum) * 3.0f);
+        const float mse = (float)sse / (std::max(1.0f, mask_sum) * 3.0f);
         packed_psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
     }
     if (res && res->final_render) {

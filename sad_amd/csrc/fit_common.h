@@ -26,6 +26,7 @@ int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const floa
 int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
                 float, uint32_t, uint32_t, int, int, bool, hipStream_t);
 int launch_clear(float* const*, int, uint32_t, hipStream_t);
+int launch_clear64(double* const*, int, uint32_t, hipStream_t);
 int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, uint32_t*, float, int, int, int, SadSite*,
                     SadAdamState*, float* const[10], float, float, float, float, float, float, float, float, float,
                     uint32_t, uint32_t, int, int, hipStream_t);
@@ -34,6 +35,7 @@ int launch_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const f
 int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float* const[8], uint32_t, const float*,
                  int, int, hipStream_t);
 int launch_sse(const float*, const float*, const float*, float*, int, int, hipStream_t);
+int launch_sse64(const float*, const float*, const float*, double*, int, int, hipStream_t);
 int launch_count_active(const SadSite*, uint32_t*, uint32_t, hipStream_t);
 int launch_init_gradient(SadSite*, uint32_t, uint32_t*, const float*, const float*, float, uint32_t, float, float, int,
                          int, hipStream_t);
@@ -104,6 +106,8 @@ int launch_gradients_aos(const uint32_t*, const uint32_t*, const float*, const f
                          float ssim_weight = 0.0f);
 int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
                 hipStream_t stream);
+int launch_ssim64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
+                  hipStream_t stream);
 int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int height, SadInferenceSite* out,
                           SadInferenceQuant* quant, hipStream_t stream);
 int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
@@ -113,6 +117,21 @@ int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant*
 int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
                         SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
                         uint32_t, int, int, hipStream_t, uint32_t* nbr_lists = nullptr, bool lists_built = false);
+/* the same tail on double [N][16] lines; requires nbr_lists */
+int launch_tau_adam_lines64(const uint32_t*, const uint32_t*, double*, float*, float*, float*, float, int, int, int,
+                            SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
+                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists, bool lists_built);
+/* the fit's double cross-tile sums: gradient lines and removal deltas, site statistics */
+int launch_gradients_lines64(const uint32_t*, const uint32_t*, const float*, const float*, double*, const SadSite*,
+                             float, uint32_t, double*, bool, int, int, int, int, hipStream_t,
+                             const float* rendered = nullptr, float ssim_weight = 0.0f);
+int launch_site_stats64(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
+                        double* const[8], int, int, int, int, hipStream_t);
+int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const double* const[8], uint32_t, const float*,
+                 int, int, hipStream_t);
+int launch_densify_pairs(const SadSite*, const double*, const double*, SadPair*, uint32_t, float, float, uint32_t,
+                         hipStream_t);
+int launch_prune_pairs(const SadSite*, const double*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
 /* the tau passes' neighbour lists of the n sites on field (c0, c1), for lambda > 0 */
 int launch_tau_lists(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, uint32_t n, float lambda, int cw,
                      int ch, int ds, uint32_t* nbr_lists, hipStream_t stream);

@@ -8,23 +8,27 @@
  * an L2.  Stages:
  *   1. each thread (one pixel) loads its 8 candidate ids and inserts them into
  *      an LDS hash of H slots (multiplicative hash, linear probing, CAS only on
- *      an empty slot).  Ids that find no slot within 8 probes are "overflow".
+ *      an empty slot).  Ids that find no slot within 8 probes are "overflow";
+ *      in the backward modes such a tile then inserts again, one wave at a time,
+ *      so that which ids overflow does not depend on wave timing.
  *   2. one thread per occupied slot loads the 40-byte site once and stores the
  *      per-site terms of the forward (tau, e^a, 1/e^a, r/S, colour) in LDS; the
  *      occupied slots are compacted into a dense list.
  *   3. each thread computes its pixel's softmax over the 8 candidates from the
  *      staged terms.  The render writes the colour here.  The backward kernels
  *      store the weights and the pixel's (pred, dL/dpred) in LDS and take a rank
- *      per (slot, pixel) with an integer LDS atomic; overflow candidates are
- *      differentiated right here and go to global memory with float atomics.
- *   4. a scan of the per-slot counts turns ranks into positions of a per-slot
- *      entry list (pixel, candidate rank), and cuts every slot's list into
- *      chunks of CHUNK entries.
+ *      per (slot, pixel) with an integer LDS atomic on a per-wave counter;
+ *      overflow candidates are differentiated right here and go to global
+ *      memory with atomics.
+ *   4. a scan of the per-slot, per-wave counts turns ranks into positions of a
+ *      per-slot entry list (pixel, candidate rank) whose order does not depend
+ *      on wave timing, and cuts every slot's list into chunks of CHUNK entries.
  *   5. chunk-parallel accumulation: CLANES (4) lanes per chunk differentiate
  *      its (pixel, candidate) pairs and sum in registers; a DPP reduction over
  *      the lane group leaves one partial per chunk in LDS; the partials of each slot are then
- *      summed (one thread per site and parameter) into one global float atomic
- *      per (site, parameter) and tile.  Chunking balances the work: the entry
+ *      summed (one thread per site and parameter) into one global atomic per
+ *      (site, parameter) and tile: float for the ABI's float outputs, double
+ *      for the fit, whose sums are then independent of the tiles' order.  Chunking balances the work: the entry
  *      counts of a tile's sites range from a few to all 256 pixels, and giving
  *      whole slots to lane groups left most lanes idle behind the largest one.
  * Why not LDS float atomics (the reference's tile hash, gradients.cu:422-459):
@@ -100,9 +104,13 @@ template <int H, bool REMOVAL, int NV>
 struct TileLds : TileSites<H> {
     static constexpr uint32_t MAX_CHUNKS = 2048u / CHUNK + (uint32_t)H;
     uint32_t start[H];         /* first entry of each slot in `list` (first: overflow site ids) */
+    /* entries per slot counted per wave, two 16-bit fields per word (waves 2v and 2v+1 in wcount[v]); after
+     * the scan, each field holds the offset of that wave's entries within the slot's list */
+    uint32_t wcount[2][H];
     uint16_t cstart[H];        /* first chunk of each slot */
     uint32_t n_chunks;
     uint32_t n_ovf;            /* pairs whose id found no hash slot */
+    uint32_t hstate;           /* nonzero: an id found no slot in its 8 probes */
     /* per pixel: slot of each candidate rank (SLOT8_NONE if unused); read only to sum a site's weight over
      * its ranks in a pixel (removal deltas, statistics) */
     uint8_t slot[(REMOVAL || NV == 8) ? 256 : 1][8];
@@ -132,16 +140,17 @@ __device__ __forceinline__ SiteTerms site_terms(const SadSite* __restrict__ site
     return t;
 }
 
+/* Insert-if-absent with linear probing, 8 probes: the slot, or SLOT_OVERFLOW. */
 template <class LDS, int H>
 __device__ __forceinline__ uint16_t hash_insert(LDS& L, uint32_t id)
 {
     uint32_t h = (id * 2654435761u) & (uint32_t)(H - 1);
 #pragma unroll 1
     for (int probe = 0; probe < 8; ++probe) {
-        uint32_t k = L.key[h];
+        const uint32_t k = L.key[h];
         if (k == id) return (uint16_t)h;
         if (k == SAD_EMPTY) {
-            uint32_t old = atomicCAS(&L.key[h], SAD_EMPTY, id);
+            const uint32_t old = atomicCAS(&L.key[h], SAD_EMPTY, id);
             if (old == SAD_EMPTY || old == id) return (uint16_t)h;
         }
         h = (h + 1u) & (uint32_t)(H - 1);
@@ -182,8 +191,11 @@ struct PixelArgs {
     float ssim_weight;         /* GRAD: weight of the -SSIM loss term */
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3] */
-    float* g[10];              /* value j of site i accumulates at g[j] + i * gstride (LINES = false) */
-    float* gline;              /* LINES = true: value j of site i accumulates at gline + i * gstride + j */
+    float* g[10];              /* DST_SOA: value j of site i accumulates at g[j] + i * gstride */
+    float* gline;              /* DST_LINES: value j of site i accumulates at gline + i * gstride + j */
+    double* gline64;           /* DST_LINES64: value j of site i at gline64 + i * 16 + j */
+    double* g64[8];            /* DST_SOA64: value j of site i at g64[j] + i */
+    double* removal64;         /* DST_LINES64 with removal: removal delta of site i at removal64[i] */
     uint32_t gstride;          /* 1: SoA arrays; 10: AoS [N][10] (BWD); 16: AoS [N][16] lines */
     float* removal;            /* GRAD with removal */
     float iss, inv_scale;
@@ -196,22 +208,40 @@ template <int MODE, bool REMOVAL, int H>
 using PixelLds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>,
                                            TileLds<H, REMOVAL, NumValues<MODE, REMOVAL>::value>>::type;
 
-/* address of value j of site id: AoS lines index by arithmetic, so a PixelArgs
- * built in registers is never indexed by a lane-varying value */
-template <bool LINES>
-__device__ __forceinline__ float* grad_addr(const PixelArgs& A, uint32_t j, uint32_t id)
+/* Where a tile's per-site sums go.  The fit accumulates across tiles in
+ * double (DST_LINES64 for gradients and removal deltas, DST_SOA64 for the
+ * densify statistics) and its consumers round once to float, as the oracle
+ * rounds its double sums: the order of the cross-tile atomics then changes a
+ * result only when a double sum lies within 1e-16 of a float rounding
+ * boundary, so fits repeat run to run.  The float kinds serve the C ABI's
+ * float outputs.  AoS lines index by arithmetic, so a PixelArgs built in
+ * registers is never indexed by a lane-varying value. */
+enum PixelDst { DST_SOA = 0, DST_LINES = 1, DST_LINES64 = 2, DST_SOA64 = 3 };
+
+template <int DST>
+__device__ __forceinline__ void acc_add(const PixelArgs& A, uint32_t j, uint32_t id, float v)
 {
-    if constexpr (LINES) return A.gline + (size_t)id * A.gstride + j;
-    else return A.g[j] + (size_t)id * A.gstride;
+    if constexpr (DST == DST_LINES64) unsafeAtomicAdd(A.gline64 + (size_t)id * 16u + j, (double)v);
+    else if constexpr (DST == DST_SOA64) unsafeAtomicAdd(A.g64[j] + id, (double)v);
+    else if constexpr (DST == DST_LINES) unsafeAtomicAdd(A.gline + (size_t)id * A.gstride + j, v);
+    else unsafeAtomicAdd(A.g[j] + (size_t)id * A.gstride, v);
+}
+
+template <int DST>
+__device__ __forceinline__ void removal_add(const PixelArgs& A, uint32_t id, float v)
+{
+    if constexpr (DST == DST_LINES64) unsafeAtomicAdd(A.removal64 + id, (double)v);
+    else unsafeAtomicAdd(A.removal + id, v);
 }
 
 /* One 16x16 tile (index `tile` of A's image) of any mode; 256 threads, one pixel each. */
-template <int MODE, bool REMOVAL, int H, bool LINES>
+template <int MODE, bool REMOVAL, int H, int DST>
 __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t tile, PixelLds<MODE, REMOVAL, H>& L)
 {
     constexpr int NV = NumValues<MODE, REMOVAL>::value;
     static_assert(H <= 128, "per-pixel slot table holds 7-bit slots");
     const uint32_t tid = threadIdx.x;
+    const uint32_t wv = tid >> 6;   /* wave of the block */
     const int tx0 = (int)(tile % A.tiles_x) * 16, ty0 = (int)(tile / A.tiles_x) * 16;
     const int x = tx0 + (int)(tid & 15u);
     const int y = ty0 + (int)(tid >> 4);
@@ -222,9 +252,16 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
         L.key[s] = SAD_EMPTY;
         L.count[s] = 0u;
+        if constexpr (MODE != MODE_RENDER) {
+            L.wcount[0][s] = 0u;
+            L.wcount[1][s] = 0u;
+        }
     }
     if constexpr (MODE != MODE_RENDER) {
-        if (tid == 0u) L.n_ovf = 0u;
+        if (tid == 0u) {
+            L.n_ovf = 0u;
+            L.hstate = 0u;
+        }
     }
     __syncthreads();
 
@@ -243,9 +280,37 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         ids[0] = a.x; ids[1] = a.y; ids[2] = a.z; ids[3] = a.w;
         ids[4] = b.x; ids[5] = b.y; ids[6] = b.z; ids[7] = b.w;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
+        for (int k = 0; k < 8; ++k)
+            slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
+    }
+    if constexpr (MODE != MODE_RENDER) {
+        bool ovf = false;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ovf = ovf || slots[k] == SLOT_OVERFLOW;
+        if (ovf) L.hstate = 1u;
     }
     __syncthreads();
+
+    /* In the backward modes a site's sum over slot entries rounds differently
+     * from its sum over direct pairs, so which ids get a slot must not depend
+     * on the order in which the waves insert.  When some id missed its probes,
+     * the tile inserts again one wave at a time: within a wave the inserts of
+     * rank k precede those of rank k + 1, and the lanes of one LDS atomic are
+     * served in a fixed order. */
+    if constexpr (MODE != MODE_RENDER) {
+        if (L.hstate) {
+            for (uint32_t s = tid; s < (uint32_t)H; s += 256u) L.key[s] = SAD_EMPTY;
+            __syncthreads();
+            for (uint32_t turn = 0; turn < 4u; ++turn) {
+                if (wv == turn) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
+                }
+                __syncthreads();
+            }
+        }
+    }
 
     /* stage 2: per-site terms of the occupied slots; dense list of them */
     for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
@@ -353,6 +418,11 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     L.px[tid] = make_float4(pr, pgr, pb, err);
     L.pg[tid] = make_float4(gr, gg, gb, 0.0f);
     if (REMOVAL) L.pt[tid] = make_float4(tr, tg, tb, 0.0f);
+    /* An entry's place in its slot's list is (entries of earlier waves) + (its rank among this wave's).  The
+     * per-wave counters make the list order, and with it every float sum over the list, independent of the
+     * order in which the tile's waves reach this point: within one wave the atomics run in program order,
+     * and the lanes of one LDS atomic instruction are served in a fixed order. */
+    const uint32_t wshift = 16u * (wv & 1u);
     uint32_t rank[8];
     uint8_t sl[8];
 #pragma unroll
@@ -361,7 +431,8 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         const uint16_t s = slots[k];
         const bool use = contrib && valid[k];
         sl[k] = (use && s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
-        rank[k] = (sl[k] != SLOT8_NONE) ? atomicAdd(&L.count[s], 1u) : 0u;
+        rank[k] = (sl[k] != SLOT8_NONE) ? ((atomicAdd(&L.wcount[wv >> 1][s], 1u << wshift) >> wshift) & 0xFFFFu)
+                                        : 0u;
         if constexpr (MODE == MODE_STATS || REMOVAL) L.slot[tid][k] = sl[k];
         if (use && s == SLOT_OVERFLOW) L.list[atomicAdd(&L.n_ovf, 1u)] = (uint16_t)((tid << 3) | (uint32_t)k);
     }
@@ -431,8 +502,8 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
                 const float v = stage[q];
                 if (v != 0.0f) {
                     const uint32_t id = L.start[e];
-                    if (MODE == MODE_GRAD && REMOVAL && j == 10u) unsafeAtomicAdd(A.removal + id, v);
-                    else unsafeAtomicAdd(grad_addr<LINES>(A, j, id), v);
+                    if (MODE == MODE_GRAD && REMOVAL && j == 10u) removal_add<DST>(A, id, v);
+                    else acc_add<DST>(A, j, id, v);
                 }
             }
             __syncthreads();
@@ -445,7 +516,13 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         uint32_t run = 0u, crun = 0u;
         for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
             const uint32_t s = s0 + tid;
-            const uint32_t c = L.count[s];
+            /* per-wave counts (at most 512 each) -> per-wave offsets within the slot, and the total */
+            const uint32_t w01 = L.wcount[0][s], w23 = L.wcount[1][s];
+            const uint32_t c0 = w01 & 0xFFFFu, c01 = c0 + (w01 >> 16), c012 = c01 + (w23 & 0xFFFFu);
+            const uint32_t c = c012 + (w23 >> 16);
+            L.wcount[0][s] = c0 << 16;
+            L.wcount[1][s] = c01 | (c012 << 16);
+            L.count[s] = c;
             const uint32_t nc = (c + CHUNK - 1u) / CHUNK;
             uint32_t v = c, vc = nc;
 #pragma unroll
@@ -471,7 +548,8 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             bool first = true;
 #pragma unroll
             for (int j = 0; j < k; ++j) first = first && (sl[j] != s);
-            L.list[L.start[s] + rank[k]] = (uint16_t)((tid << 3) | (uint32_t)k | (first ? ENTRY_FIRST : 0u));
+            const uint32_t at = L.start[s] + ((L.wcount[wv >> 1][s] >> wshift) & 0xFFFFu) + rank[k];
+            L.list[at] = (uint16_t)((tid << 3) | (uint32_t)k | (first ? ENTRY_FIRST : 0u));
         }
     }
     __syncthreads();
@@ -549,19 +627,19 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         for (uint32_t q = q0; q < q1; ++q) v += L.part[q][j];
         const uint32_t id = L.key[s];
         if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
-            if (v != 0.0f) unsafeAtomicAdd(A.removal + id, v);
+            if (v != 0.0f) removal_add<DST>(A, id, v);
         } else {
-            unsafeAtomicAdd(grad_addr<LINES>(A, j, id), v);
+            acc_add<DST>(A, j, id, v);
         }
     }
     }  /* backward modes */
 }
 
-template <int MODE, bool REMOVAL, int H, bool LINES>
+template <int MODE, bool REMOVAL, int H, int DST>
 __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
     __shared__ PixelLds<MODE, REMOVAL, H> L;
-    pixel_tile<MODE, REMOVAL, H, LINES>(A, xcd_tile(blockIdx.x, A.ntiles), L);
+    pixel_tile<MODE, REMOVAL, H, DST>(A, xcd_tile(blockIdx.x, A.ntiles), L);
 }
 
 /* Batched fit: one launch over the concatenated tiles of every job. */
@@ -579,9 +657,9 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.sites = J.sites;
     A.target = J.target;
     A.mask = J.mask;
-    A.gline = J.gacc;
+    A.gline64 = J.gacc;
+    A.removal64 = J.removal;
     A.gstride = 16u;
-    A.removal = J.removal;
     A.iss = J.iss;
     A.inv_scale = J.inv_scale;
     A.n = J.n;
@@ -591,7 +669,7 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.ch = J.ch;
     A.tiles_x = J.tiles_x;
     A.ntiles = J.ntiles;
-    pixel_tile<MODE_GRAD, REMOVAL, H, true>(A, f - J.tile_begin, L);
+    pixel_tile<MODE_GRAD, REMOVAL, H, DST_LINES64>(A, f - J.tile_begin, L);
 }
 
 /* ------------------------------------------------------------ launchers -- */
@@ -601,13 +679,13 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
 #endif
 constexpr int HASH_SLOTS = SAD_HASH_SLOTS;
 
-template <int MODE, bool REMOVAL, bool LINES = false>
+template <int MODE, bool REMOVAL, int DST = DST_SOA>
 static int launch_pixel(PixelArgs& A, hipStream_t stream, const char* what)
 {
     A.tiles_x = div_up((uint32_t)A.width, 16u);
     A.ntiles = A.tiles_x * div_up((uint32_t)A.height, 16u);
     A.inv_scale = sqrtf(A.iss);
-    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS, LINES>), dim3(A.ntiles), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS, DST>), dim3(A.ntiles), dim3(256), 0, stream, A);
     return launch_status(what);
 }
 
@@ -649,8 +727,26 @@ int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const flo
     A.gstride = 16u;
     A.removal = removal; A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    if (compute_removal) return launch_pixel<MODE_GRAD, true, true>(A, stream, "compute_gradients_tiled(aos)");
-    return launch_pixel<MODE_GRAD, false, true>(A, stream, "compute_gradients_tiled(aos)");
+    if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_LINES>(A, stream, "compute_gradients_tiled(aos)");
+    return launch_pixel<MODE_GRAD, false, DST_LINES>(A, stream, "compute_gradients_tiled(aos)");
+}
+
+/* the fit's gradients: double AoS [N][16] lines and double removal deltas */
+int launch_gradients_lines64(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                             double* gacc, const SadSite* sites, float iss, uint32_t n, double* removal,
+                             bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
+                             const float* rendered, float ssim_weight)
+{
+    PixelArgs A = {};
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
+    A.rendered = rendered; A.ssim_weight = ssim_weight;
+    A.gline64 = gacc;
+    A.removal64 = removal;
+    A.gstride = 16u;
+    A.iss = iss; A.n = n;
+    A.width = width; A.height = height; A.cw = cw; A.ch = ch;
+    if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_LINES64>(A, stream, "compute_gradients(lines64)");
+    return launch_pixel<MODE_GRAD, false, DST_LINES64>(A, stream, "compute_gradients(lines64)");
 }
 
 int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
@@ -676,6 +772,19 @@ int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float*
     A.gstride = 1u;
     A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     return launch_pixel<MODE_STATS, false>(A, stream, "compute_site_stats");
+}
+
+/* the fit's densify statistics, accumulated in double */
+int launch_site_stats64(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                        const SadSite* sites, float iss, uint32_t n, double* const st[8], int width, int height,
+                        int cw, int ch, hipStream_t stream)
+{
+    PixelArgs A = {};
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
+    for (int j = 0; j < 8; ++j) A.g64[j] = st[j];
+    A.gstride = 1u;
+    A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
+    return launch_pixel<MODE_STATS, false, DST_SOA64>(A, stream, "compute_site_stats(double)");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream);
@@ -708,7 +817,7 @@ int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const flo
     A.gstride = 10u;
     A.iss = inv_scale_sq; A.n = site_count; A.width = width; A.height = height;
     A.cw = cand_width; A.ch = cand_height;
-    return launch_pixel<MODE_BWD, false, true>(A, to_hip(stream), "render_backward");
+    return launch_pixel<MODE_BWD, false, DST_LINES>(A, to_hip(stream), "render_backward");
 }
 
 int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, const float* target,

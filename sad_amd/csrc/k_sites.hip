@@ -21,6 +21,7 @@ struct AdamArgs {
     SadAdamState* st;
     float* g[10];              /* gradient j of site i at g[j][i * gstride] */
     uint32_t gstride;          /* 1: SoA arrays; 16: AoS [N][16] lines */
+    double* g64;               /* non-null: gradient j of site i at g64[i * 16 + j] instead (double lines) */
     float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
     float beta1, beta2, eps, b1c, b2c, norm, maxx, maxy;
     uint32_t n;
@@ -87,6 +88,33 @@ __device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const f
 
     sp[0] = s0; sp[1] = s1; sp[2] = s2; sp[3] = s3; sp[4] = s4;
     ap[0] = a0; ap[1] = a1; ap[2] = a2; ap[3] = a3; ap[4] = a4;
+}
+
+/* site i's ten gradients times norm, with log_tau replaced by `tau`, then
+ * zeroed at the source.  Double lines round once to float here: the
+ * cross-tile sums that filled them are exact to well below float precision,
+ * so the float seen by Adam does not depend on the order of the atomics. */
+__device__ __forceinline__ void take_gradients(const AdamArgs& A, uint32_t i, float tau, float g[10])
+{
+    if (A.g64) {
+        double2* line = reinterpret_cast<double2*>(A.g64 + (size_t)i * 16u);
+        double v[10];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const double2 d = line[j];
+            v[2 * j] = d.x;
+            v[2 * j + 1] = d.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? tau : (float)v[j]) * A.norm;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) line[j] = make_double2(0.0, 0.0);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? tau : A.g[j][i * A.gstride]) * A.norm;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
 }
 
 template <bool CLEAR>
@@ -166,18 +194,19 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
  * tau_site, so the results are bit-identical. */
 #define NBR_COPY 0xFFFFFFFEu
 
+template <typename TR, typename TI>
 __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                              const SadSite* __restrict__ sites, const float* __restrict__ graw,
-                                              const float* __restrict__ gin, float* __restrict__ gout,
+                                              const SadSite* __restrict__ sites, const TR* __restrict__ graw,
+                                              const TI* __restrict__ gin, float* __restrict__ gout,
                                               float* __restrict__ raw_copy, uint32_t n, float lambda, int width,
                                               int height, float inv_ds, uint32_t in_stride, uint4* __restrict__ nbr)
 {
-    const float raw = graw[i * in_stride];
+    const float raw = (float)graw[i * in_stride];
     if (raw_copy) raw_copy[i] = raw;
     const float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
     uint32_t keep[8];
     if (lambda <= 0.0f || pos.x < 0.0f) {
-        gout[i] = gin[i * in_stride];
+        gout[i] = (float)gin[i * in_stride];
         nbr[2u * i] = make_uint4(NBR_COPY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
         nbr[2u * i + 1u] = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
         return;
@@ -197,7 +226,7 @@ __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restric
         ok = ok && sites[nb].pos_x >= 0.0f;
         keep[k] = ok ? nb : SAD_EMPTY;
         if (ok) {
-            sum += gin[nb * in_stride];
+            sum += (float)gin[nb * in_stride];
             cnt += 1u;
         }
     }
@@ -236,22 +265,23 @@ __device__ __forceinline__ void tau_list_only(uint32_t i, const uint4* __restric
 }
 
 /* one pass from the list: the value tau_site computes (graw and gin hold site i at [i * stride]) */
-__device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const float* __restrict__ graw,
-                                               const float* __restrict__ gin, float lambda, uint32_t stride = 1u)
+template <typename TR, typename TI>
+__device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const TR* __restrict__ graw,
+                                               const TI* __restrict__ gin, float lambda, uint32_t stride = 1u)
 {
     const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
-    if (a.x == NBR_COPY) return gin[i * stride];
+    if (a.x == NBR_COPY) return (float)gin[i * stride];
     const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     float sum = 0.0f;
     uint32_t cnt = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (ids[k] != SAD_EMPTY) {
-            sum += gin[ids[k] * stride];
+            sum += (float)gin[ids[k] * stride];
             cnt += 1u;
         }
     }
-    return (graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
+    return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
 __global__ __launch_bounds__(256) void k_tau_lists(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
@@ -263,18 +293,20 @@ __global__ __launch_bounds__(256) void k_tau_lists(const uint4* __restrict__ c0,
 }
 
 /* first pass from built lists, reading the log_tau column of the gradient lines; saves raw */
+template <typename T>
 __global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __restrict__ nbr,
-                                                              const float* __restrict__ g2, float* __restrict__ gout,
+                                                              const T* __restrict__ g2, float* __restrict__ gout,
                                                               float* __restrict__ raw_copy, uint32_t n, float lambda)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    raw_copy[i] = g2[i * 16u];
+    raw_copy[i] = (float)g2[i * 16u];
     gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                        const SadSite* __restrict__ sites, const float* __restrict__ g2,
+                                                        const SadSite* __restrict__ sites, const T* __restrict__ g2,
                                                         float* __restrict__ gout, float* __restrict__ raw_copy,
                                                         uint32_t n, float lambda, int width, int height, float inv_ds,
                                                         uint4* __restrict__ nbr, uint32_t in_stride)
@@ -298,12 +330,8 @@ __global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restr
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= A.n) return;
-    const float v = tau_from_list(i, nbr, graw, gin, lambda);
     float g[10];
-#pragma unroll
-    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
-#pragma unroll
-    for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+    take_gradients(A, i, tau_from_list(i, nbr, graw, gin, lambda), g);
     adam_site(A, i, g);
 }
 
@@ -369,10 +397,7 @@ __device__ __forceinline__ void tau_last_adam_site(uint32_t i, const uint4* __re
         v = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
     }
     float g[10];
-#pragma unroll
-    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
-#pragma unroll
-    for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+    take_gradients(A, i, v, g);
     adam_site(A, i, g);
 }
 
@@ -411,18 +436,12 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
     } else {
         AdamArgs A;
         A.sites = J.sites; A.st = J.adam;
-#pragma unroll
-        for (int k = 0; k < 10; ++k) A.g[k] = J.gacc + k;
-        A.gstride = 16u;
+        A.g64 = J.gacc;
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
         A.b1c = js.b1c; A.b2c = js.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
-        const float v = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda);
         float g[10];
-#pragma unroll
-        for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? v : A.g[j][i * A.gstride]) * A.norm;
-#pragma unroll
-        for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+        take_gradients(A, i, tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda), g);
         adam_site(A, i, g);
     }
 }
@@ -448,15 +467,16 @@ __global__ void k_update_jobs(BatchJob* jobs, JobUpdate u)
 
 /* computeDensifyScorePairsKernel, densify_prune.cu:223-254.  pow(m, alpha) is
  * the exact exp(alpha * log m) of sad_math.h so keys match the CPU bit for bit. */
-__global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict__ sites, const float* __restrict__ mass,
-                                                       const float* __restrict__ energy, SadPair* __restrict__ pairs,
+template <typename T>
+__global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict__ sites, const T* __restrict__ mass,
+                                                       const T* __restrict__ energy, SadPair* __restrict__ pairs,
                                                        uint32_t n, float min_mass, float alpha, uint32_t count)
 {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= count) return;
     uint2 out = make_uint2(SAD_EMPTY, SAD_EMPTY);
     if (i < n) {
-        float m = mass[i], e = energy[i];
+        const float m = (float)mass[i], e = (float)energy[i];
         if (sites[i].pos_x >= 0.0f && m > min_mass && isfinite(e)) {
             float den = sad_powf_pos(fmaxf(m, 1e-8f), alpha);
             float score = fmaxf(e, 0.0f) / den;
@@ -467,7 +487,8 @@ __global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict
 }
 
 /* computePruneScorePairsKernel, densify_prune.cu:257-281 */
-__global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__ sites, const float* __restrict__ rdelta,
+template <typename T>
+__global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__ sites, const T* __restrict__ rdelta,
                                                      SadPair* __restrict__ pairs, uint32_t n, float delta_norm,
                                                      uint32_t count)
 {
@@ -475,7 +496,7 @@ __global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__
     if (i >= count) return;
     uint2 out = make_uint2(SAD_EMPTY, SAD_EMPTY);
     if (i < n) {
-        float d = rdelta[i] * delta_norm;
+        float d = (float)rdelta[i] * delta_norm;
         if (sites[i].pos_x >= 0.0f && isfinite(d)) out = make_uint2(__float_as_uint(fmaxf(d, 0.0f)), i);
     }
     reinterpret_cast<uint2*>(pairs)[i] = out;
@@ -501,17 +522,19 @@ __global__ __launch_bounds__(256) void k_prune_sites(SadSite* __restrict__ sites
 }
 
 /* splitSitesKernel, densify_prune.cu:86-220, with parents >= cur skipped */
+template <typename T>
 struct SplitArgs {
     SadSite* sites;
     SadAdamState* st;
     const uint32_t* idx;
-    const float *mass, *ew, *ewx, *ewy, *ewxx, *ewxy, *ewyy;
+    const T *mass, *ew, *ewx, *ewy, *ewxx, *ewxy, *ewyy;
     const float* target;
     uint32_t k, cur;
     int width, height;
 };
 
-__global__ __launch_bounds__(256) void k_split(SplitArgs A)
+template <typename T>
+__global__ __launch_bounds__(256) void k_split(SplitArgs<T> A)
 {
     uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     if (gid >= A.k) return;
@@ -519,18 +542,18 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs A)
     if (pi >= A.cur) return;
     SadSite parent = A.sites[pi];
     if (parent.pos_x < 0.0f) return;
-    float w = A.ew[pi];
+    const float w = (float)A.ew[pi];
     float ax = parent.dir_x, ay = parent.dir_y;
     float cx = parent.pos_x, cy = parent.pos_y;
     float la = parent.log_aniso * 0.8f;
     const int W = A.width, Hh = A.height;
     if (w > 1e-3f) {
-        float mx = A.ewx[pi] / w, my = A.ewy[pi] / w;
+        float mx = (float)A.ewx[pi] / w, my = (float)A.ewy[pi] / w;
         cx = 0.4f * parent.pos_x + 0.6f * mx;
         cy = 0.4f * parent.pos_y + 0.6f * my;
-        float exx = fmaxf(A.ewxx[pi] / w - mx * mx, 1e-4f);
-        float exy = A.ewxy[pi] / w - mx * my;
-        float eyy = fmaxf(A.ewyy[pi] / w - my * my, 1e-4f);
+        float exx = fmaxf((float)A.ewxx[pi] / w - mx * mx, 1e-4f);
+        float exy = (float)A.ewxy[pi] / w - mx * my;
+        float eyy = fmaxf((float)A.ewyy[pi] / w - my * my, 1e-4f);
         float th = 0.5f * atan2f(2.0f * exy, exx - eyy);
         ax = cosf(th);
         ay = sinf(th);
@@ -559,7 +582,7 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs A)
             if (len > 1e-8f) { ax = gx / len; ay = gy / len; } else { ax = 1.0f; ay = 0.0f; }
         }
     }
-    float m = fmaxf(A.mass[pi], 1.0f);
+    float m = fmaxf((float)A.mass[pi], 1.0f);
     float od = sad_clampf(sqrtf(m) * 0.5f, 1.5f, 48.0f);
     float ox = ax * od, oy = ay * od;
     float pax = sad_clampf(cx + ox, 0.0f, (float)(W - 1)), pay = sad_clampf(cy + oy, 0.0f, (float)(Hh - 1));
@@ -619,10 +642,13 @@ __global__ __launch_bounds__(256) void k_hilbert_order(const SadPair* __restrict
 /* ------------------------------------------------------------- metrics --- */
 
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_dword(unsigned long long* p) { *p = 0ull; }
 
 /* computePSNRKernel, utils.cu:5-26: block sum of squared error, one atomic per block */
+/* T = double: the block partials meet in a double sum, which does not depend on the blocks' order */
+template <typename T>
 __global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const float* __restrict__ t,
-                                             const float* __restrict__ mask, float* __restrict__ acc, uint32_t np)
+                                             const float* __restrict__ mask, T* __restrict__ acc, uint32_t np)
 {
     __shared__ float part[4];
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -634,13 +660,14 @@ __global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const 
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)((part[0] + part[1]) + (part[2] + part[3])));
 }
 
 /* computeSSIMKernel, utils.cu:29-87: sum over masked pixels of the SSIM of the
  * pixel's 3x3 luma window; 16x16 tiles so the windows overlap in cache */
+template <typename T>
 __global__ __launch_bounds__(256) void k_ssim(const float* __restrict__ r, const float* __restrict__ t,
-                                              const float* __restrict__ mask, float* __restrict__ acc, int width,
+                                              const float* __restrict__ mask, T* __restrict__ acc, int width,
                                               int height, uint32_t tiles_x)
 {
     __shared__ float part[4];
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(256) void k_ssim(const float* __restrict__ r, const
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (part[0] + part[1]) + (part[2] + part[3]));
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)((part[0] + part[1]) + (part[2] + part[3])));
 }
 
 /* -------------------------------------------- quantised inference sites --
@@ -870,6 +897,7 @@ static void fill_adam(AdamArgs& A, SadSite* sites, SadAdamState* st, float* cons
     A.sites = sites; A.st = st;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
     A.gstride = 1u;
+    A.g64 = nullptr;
     A.lr_pos = lr_pos; A.lr_tau = lr_tau; A.lr_radius = lr_radius; A.lr_color = lr_color;
     A.lr_dir = lr_dir; A.lr_aniso = lr_aniso; A.beta1 = beta1; A.beta2 = beta2; A.eps = eps;
     A.b1c = 1.0f / (1.0f - powf(beta1, (float)t));
@@ -909,11 +937,13 @@ int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* t
     if (n == 0) return SADH_OK;
     const dim3 grid(div_up(n, 256u));
     const uint4* nbr = reinterpret_cast<const uint4*>(nbr_lists);
-    hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
+    hipLaunchKernelGGL(k_tau_list_first<float>, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
                        reinterpret_cast<const uint4*>(c1), sites, g[2], tmp, raw, n, lambda, cand_width, cand_height,
                        1.0f / fmaxf((float)ds, 1.0f), reinterpret_cast<uint4*>(nbr_lists), 1u);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, raw, tmp, g[2], n, lambda);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, raw, g[2], tmp, n, lambda);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, (const float*)raw, (const float*)tmp, g[2],
+                       n, lambda);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, (const float*)raw, (const float*)g[2], tmp,
+                       n, lambda);
     AdamArgs A;
     fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
               height);
@@ -948,10 +978,37 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
     return launch_status("batch_update_jobs");
 }
 
-/* The fit's tail on AoS [N][16] gradient lines: pass 1 reads the strided
- * log_tau column (raw saved), passes 2-3 ping-pong tmp -> tmp2 -> tmp, pass 4
- * feeds Adam, which reads and clears each site's line.  Same values as the
- * SoA chain (the buffers differ, not the arithmetic). */
+/* The list-driven tail on [N][16] gradient lines of type T: pass 1 reads the
+ * strided log_tau column (raw saved) and, unless lists_built, writes the
+ * neighbour lists; passes 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam,
+ * which reads and clears each site's line. */
+template <typename T>
+static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw, float* tmp, float* tmp2,
+                           float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
+                           uint32_t* nbr_lists, bool lists_built, const AdamArgs& A, hipStream_t stream)
+{
+    const uint32_t n = A.n;
+    const dim3 grid(div_up(n, 256u));
+    uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
+    const T* g2 = gacc + 2;
+    if (lists_built)
+        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(256), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
+                           lambda);
+    else
+        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
+                           raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+                       (const float*)tmp, tmp2, n, lambda);
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+                       (const float*)tmp2, tmp, n, lambda);
+    hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+                       (const float*)tmp, lambda, A);
+}
+
+/* The fit's tail on AoS [N][16] float gradient lines: with neighbour lists as
+ * tau_adam_lines, else four passes that each walk the candidate field, the
+ * last fused with Adam.  Same values as the SoA chain (the buffers differ, not
+ * the arithmetic). */
 int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, float* raw, float* tmp, float* tmp2,
                         float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                         float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
@@ -970,16 +1027,8 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
               height);
     A.gstride = 16u;
     if (nbr_lists) {
-        uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
-        if (lists_built)
-            hipLaunchKernelGGL(k_tau_list_first_built, grid, dim3(256), 0, stream, (const uint4*)nbr, gacc + 2, tmp,
-                               raw, n, lambda);
-        else
-            hipLaunchKernelGGL(k_tau_list_first, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n,
-                               lambda, cand_width, cand_height, inv_ds, nbr, 16u);
-        hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, tmp2, n, lambda);
-        hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp2, tmp, n, lambda);
-        hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, raw, tmp, lambda, A);
+        tau_adam_lines(u0, u1, gacc, raw, tmp, tmp2, lambda, cand_width, cand_height, inv_ds, sites, nbr_lists,
+                       lists_built, A, stream);
         return launch_status("tau_diffuse_adam(aos, lists)");
     }
     hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
@@ -991,6 +1040,27 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
     hipLaunchKernelGGL(k_tau_last_adam, grid, dim3(256), 0, stream, u0, u1, raw, tmp, lambda, cand_width,
                        cand_height, inv_ds, A);
     return launch_status("tau_diffuse_adam(aos)");
+}
+
+/* The same tail on double gradient lines (the fit's cross-tile sums), which
+ * always runs from neighbour lists (nbr_lists: 8 uint32 per site). */
+int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc, float* raw, float* tmp, float* tmp2,
+                            float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
+                            float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
+                            float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
+                            hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
+{
+    if (n == 0) return SADH_OK;
+    if (!nbr_lists) return fail(SADH_ERR_INVALID, "tau_diffuse_adam(lines64): neighbour lists are required");
+    float* g[10] = {};
+    AdamArgs A;
+    fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
+              height);
+    A.g64 = gacc;
+    tau_adam_lines(reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), gacc, raw, tmp, tmp2,
+                   lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, lists_built, A,
+                   stream);
+    return launch_status("tau_diffuse_adam(lines64)");
 }
 
 int launch_tau_lists(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, uint32_t n, float lambda, int cw,
@@ -1014,6 +1084,14 @@ int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t
     return launch_status("clear");
 }
 
+/* zeroes n doubles in each array, as 2n floats (the same all-zero bits) */
+int launch_clear64(double* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
+{
+    float* f[10];
+    for (int j = 0; j < count_arrays; ++j) f[j] = reinterpret_cast<float*>(arrays[j]);
+    return launch_clear(f, count_arrays, 2u * n, stream);
+}
+
 int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, const float* graw,
                        const float* gin, float* gout, float* raw_copy, uint32_t n, float lambda, int width,
                        int height, int ds, hipStream_t stream)
@@ -1026,17 +1104,30 @@ int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* si
     return launch_status("tau_diffuse");
 }
 
-int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const float* const stats[8],
-                 uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+template <typename T>
+static int split_impl(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const T* const stats[8],
+                      uint32_t cur, const float* target, int width, int height, hipStream_t stream)
 {
     if (k == 0) return SADH_OK;
-    SplitArgs A;
+    SplitArgs<T> A;
     A.sites = sites; A.st = st; A.idx = idx;
     A.mass = stats[0]; A.ew = stats[2]; A.ewx = stats[3]; A.ewy = stats[4];
     A.ewxx = stats[5]; A.ewxy = stats[6]; A.ewyy = stats[7];
     A.target = target; A.k = k; A.cur = cur; A.width = width; A.height = height;
-    hipLaunchKernelGGL(k_split, dim3(div_up(k, 256u)), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(k_split<T>, dim3(div_up(k, 256u)), dim3(256), 0, stream, A);
     return launch_status("split_sites");
+}
+
+int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const float* const stats[8],
+                 uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+{
+    return split_impl(sites, st, idx, k, stats, cur, target, width, height, stream);
+}
+
+int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const double* const stats[8],
+                 uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+{
+    return split_impl(sites, st, idx, k, stats, cur, target, width, height, stream);
 }
 
 int launch_sse(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
@@ -1044,7 +1135,16 @@ int launch_sse(const float* rendered, const float* target, const float* mask, fl
 {
     uint32_t np = (uint32_t)width * (uint32_t)height;
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
-    hipLaunchKernelGGL(k_sse, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
+    hipLaunchKernelGGL(k_sse<float>, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
+    return launch_status("compute_psnr");
+}
+
+int launch_sse64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
+                 hipStream_t stream)
+{
+    uint32_t np = (uint32_t)width * (uint32_t)height;
+    hipLaunchKernelGGL(k_zero_dword, dim3(1), dim3(1), 0, stream, reinterpret_cast<unsigned long long*>(acc));
+    hipLaunchKernelGGL(k_sse<double>, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
     return launch_status("compute_psnr");
 }
 
@@ -1053,7 +1153,18 @@ int launch_ssim(const float* rendered, const float* target, const float* mask, f
 {
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
     const uint32_t tx = div_up((uint32_t)width, 16u), ty = div_up((uint32_t)height, 16u);
-    hipLaunchKernelGGL(k_ssim, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width, height, tx);
+    hipLaunchKernelGGL(k_ssim<float>, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width, height,
+                       tx);
+    return launch_status("compute_ssim");
+}
+
+int launch_ssim64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
+                  hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_dword, dim3(1), dim3(1), 0, stream, reinterpret_cast<unsigned long long*>(acc));
+    const uint32_t tx = div_up((uint32_t)width, 16u), ty = div_up((uint32_t)height, 16u);
+    hipLaunchKernelGGL(k_ssim<double>, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width,
+                       height, tx);
     return launch_status("compute_ssim");
 }
 
@@ -1088,22 +1199,48 @@ int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipSt
     return launch_status("count_active_sites");
 }
 
+template <typename T>
+static int densify_impl(const SadSite* sites, const T* mass, const T* energy, SadPair* pairs, uint32_t n,
+                        float min_mass, float alpha, uint32_t count, hipStream_t stream)
+{
+    if (count == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_densify_pairs<T>, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, mass, energy,
+                       pairs, n, min_mass, alpha, count);
+    return launch_status("compute_densify_score_pairs");
+}
+
 int launch_densify_pairs(const SadSite* sites, const float* mass, const float* energy, SadPair* pairs, uint32_t n,
                          float min_mass, float alpha, uint32_t count, hipStream_t stream)
 {
+    return densify_impl(sites, mass, energy, pairs, n, min_mass, alpha, count, stream);
+}
+
+int launch_densify_pairs(const SadSite* sites, const double* mass, const double* energy, SadPair* pairs, uint32_t n,
+                         float min_mass, float alpha, uint32_t count, hipStream_t stream)
+{
+    return densify_impl(sites, mass, energy, pairs, n, min_mass, alpha, count, stream);
+}
+
+template <typename T>
+static int prune_impl(const SadSite* sites, const T* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
+                      uint32_t count, hipStream_t stream)
+{
     if (count == 0) return SADH_OK;
-    hipLaunchKernelGGL(k_densify_pairs, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, mass, energy, pairs,
-                       n, min_mass, alpha, count);
-    return launch_status("compute_densify_score_pairs");
+    hipLaunchKernelGGL(k_prune_pairs<T>, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, rdelta, pairs, n,
+                       delta_norm, count);
+    return launch_status("compute_prune_score_pairs");
 }
 
 int launch_prune_pairs(const SadSite* sites, const float* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
                        uint32_t count, hipStream_t stream)
 {
-    if (count == 0) return SADH_OK;
-    hipLaunchKernelGGL(k_prune_pairs, dim3(div_up(count, 256u)), dim3(256), 0, stream, sites, rdelta, pairs, n,
-                       delta_norm, count);
-    return launch_status("compute_prune_score_pairs");
+    return prune_impl(sites, rdelta, pairs, n, delta_norm, count, stream);
+}
+
+int launch_prune_pairs(const SadSite* sites, const double* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
+                       uint32_t count, hipStream_t stream)
+{
+    return prune_impl(sites, rdelta, pairs, n, delta_norm, count, stream);
 }
 
 int launch_write_indices(const SadPair* sorted, uint32_t* idx, uint32_t count, hipStream_t stream)

@@ -64,3 +64,19 @@ def test_batch_rejects_bad_jobs():
     bad = synth.sites_array(0)
     with pytest.raises(SadHipError):
         trainer.fit_batch(tg, [bad], opts)
+
+
+def test_fits_repeat_exactly():
+    """The fit's cross-tile sums meet in fp64 and the gradient kernel's tile
+    hash holds a set of ids that does not depend on the order of its inserts,
+    so a fit with densify and prune repeats bit for bit, alone and batched."""
+    specs = [(150, 100, 62, 500), (90, 170, 63, 400)]
+    tg, st = _jobs(specs)
+    opts = config.fit_options(iterations=120, target_bpp=1.5, densify_freq=20, prune_freq=20, densify_start=20,
+                              prune_start=40, max_sites=2000, log_freq=30)
+    runs = [trainer.fit_batch(tg, st, opts)["jobs"] for _ in range(2)]
+    for i, s0 in enumerate(st):
+        single = [trainer.fit(tg[i], s0, dict(opts, n_sites=s0.shape[0])) for _ in range(2)]
+        for r in (runs[1][i], single[0], single[1]):
+            assert np.array_equal(r["sites"].view(np.uint32), runs[0][i]["sites"].view(np.uint32))
+            assert np.array_equal(np.asarray(r["log_psnr"]), np.asarray(runs[0][i]["log_psnr"]))
