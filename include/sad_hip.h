@@ -299,7 +299,7 @@ typedef struct SadhFitOptions {
     float init_gradient_alpha;        /* init_mode 2: threshold = 0.01 * alpha */
     float init_log_tau, init_radius;  /* init_mode 2: initial log_tau and radius */
     float ssim_weight;                /* > 0: loss L2 - w * SSIM, with a render before each gradient
-                                         (--ssim-weight, sad_cuda.cu:2307-2330); sadh_fit only */
+                                         (--ssim-weight, sad_cuda.cu:2307-2330) */
     int ssim_metric;                  /* 1: log the mean SSIM beside the PSNR (--ssim, sad_cuda.cu:2432-2447) */
     int packed_psnr;                  /* 1: after the fit, render the quantised inference sites with the final
                                          candidates and report that PSNR (--packed-psnr, training.swift:676-703) */
@@ -318,7 +318,7 @@ typedef struct SadhFitResult {
     float* grad_ms_trace;             /* in: optional host array; out: per-launch gradient ms */
     int grad_ms_trace_cap;            /* in: entries of grad_ms_trace */
     float* final_render;              /* in: optional host [H][W][3]; out: render of the final sites with the
-                                         final candidates (sad_cuda.cu:2482-2496); sadh_fit only */
+                                         final candidates (sad_cuda.cu:2482-2496) */
     float final_ssim, best_ssim;      /* ssim_metric: mean SSIM at the last log, and the best */
     float* log_ssim;                  /* in: optional host array of log_cap entries; out: the logged SSIMs */
     float packed_psnr;                /* packed_psnr: PSNR of the quantised inference sites */

@@ -54,6 +54,11 @@ struct FitRun {
     DevArena mem;
     float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr;
     double* d_mse = nullptr;
+    double* d_ssim = nullptr;          /* ssim_metric: sum of the per-pixel SSIM */
+    float ssim_weight = 0.0f;          /* SSIM loss weight */
+    SadInferenceSite* d_packed = nullptr;   /* packed_psnr: quantised sites, their quant, decoded sites */
+    SadInferenceQuant* d_quant = nullptr;
+    SadSite* d_dec = nullptr;
     float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr;
     double* removal = nullptr;
     uint32_t* tau_nbr = nullptr;
@@ -72,7 +77,8 @@ struct FitRun {
     uint32_t n = 0, cur = 0, hil_count = 0;
     bool hil_ready = false;
     float best = 0.0f, final_psnr = 0.0f;
-    double sse = 0.0;
+    float final_ssim = 0.0f, best_ssim = 0.0f, packed_psnr = 0.0f;
+    double sse = 0.0, ssim_sum = 0.0;
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
     double submit_s = 0.0, admit_s = 0.0;
@@ -93,6 +99,7 @@ size_t fit_bytes(const SadhFitOptions& o, int width, int height, int cap, int pa
     /* site, Adam state, double gradient line, 3 tau buffers, double removal delta, tau neighbour list, search site */
     b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 128 + 12 + 8 + 32 + sizeof(SadSearchSite));
     if (o.densify_enabled) b += (size_t)cap * 64;
+    if (o.packed_psnr) b += (size_t)cap * (sizeof(SadInferenceSite) + sizeof(SadSite)) + sizeof(SadInferenceQuant);
     if (pairs_count > 0) b += (size_t)pairs_count * 16 + 256 * 4 * (pairs_count / 1024 + 1) + 2 * 65536 * 4;
     if (use_hil) b += (size_t)cap * 8 + 16 * (size_t)(cap / 1024 + 1) * 1024 * 2;
     return b;
@@ -151,6 +158,13 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.d_mask, (size_t)J.np));
     SAD_TRY(J.mem.alloc(&J.d_render, (size_t)J.np * 3));
     SAD_TRY(J.mem.alloc(&J.d_mse, 1));
+    SAD_TRY(J.mem.alloc(&J.d_ssim, 1));
+    J.ssim_weight = base.ssim_weight > 0.0f ? base.ssim_weight : 0.0f;
+    if (base.packed_psnr) {
+        SAD_TRY(J.mem.alloc(&J.d_packed, (size_t)cap));
+        SAD_TRY(J.mem.alloc(&J.d_quant, 1));
+        SAD_TRY(J.mem.alloc(&J.d_dec, (size_t)cap));
+    }
     SAD_TRY(J.mem.alloc(&J.d_count, 1));
     SAD_TRY(J.mem.alloc(&J.d_sites, cap));
     SAD_TRY(J.mem.alloc(&J.d_adam, cap));
@@ -224,6 +238,8 @@ void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.tau_tmp2 = J.tau_tmp2;
     d.tau_nbr = J.tau_nbr;
     d.removal = J.removal;
+    d.rendered = J.d_render;
+    d.ssim_weight = J.ssim_weight;
     d.h_order = J.h_order;
     d.h_pos = J.h_pos;
     d.n = J.n;
@@ -275,8 +291,6 @@ struct Engine {
     {
         o = opts;
         if (o.init_mode != 0) return fail(SADH_ERR_UNSUPPORTED, "fit_batch: only caller-provided initial sites");
-        if (o.ssim_weight > 0.0f || o.ssim_metric || o.packed_psnr)
-            return fail(SADH_ERR_UNSUPPORTED, "fit_batch: the SSIM loss and metric and packed PSNR are sadh_fit only");
         use_hil = o.cand_hilbert_probes > 0 && o.cand_hilbert_window > 0;
         HIP_TRY(hipStreamCreateWithFlags(&sp.main, hipStreamNonBlocking));
         HIP_TRY(hipStreamCreateWithFlags(&sp.cand, hipStreamNonBlocking));
@@ -471,6 +485,13 @@ struct Engine {
             gev.push_back(b);
             HIP_TRY(hipEventRecord(a, S));
         }
+        /* the SSIM loss term reads a render of each job's current state (sad_cuda.cu:2307-2318) */
+        if (o.ssim_weight > 0.0f)
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
+                                      J.ch, S));
+            }
         SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, any_prune, S));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
@@ -513,6 +534,10 @@ struct Engine {
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
                                       J.ch, S));
                 SAD_TRY(launch_sse64(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
+                if (o.ssim_metric) {
+                    SAD_TRY(launch_ssim64(J.d_render, J.d_target, J.d_mask, J.d_ssim, J.width, J.height, S));
+                    HIP_TRY(hipMemcpyAsync(&J.ssim_sum, J.d_ssim, sizeof(double), hipMemcpyDeviceToHost, S));
+                }
                 SAD_TRY(launch_count_active(J.d_sites, J.d_count, J.n, S));
                 HIP_TRY(hipMemcpyAsync(&J.sse, J.d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
                 HIP_TRY(hipMemcpyAsync(&J.active_count, J.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
@@ -524,6 +549,13 @@ struct Engine {
                 float mse = (float)J.sse / (std::max(1.0f, J.mask_sum) * 3.0f);
                 float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
                 if (J.io.log_psnr && J.n_logs < J.io.log_cap) J.io.log_psnr[J.n_logs] = psnr;
+                if (o.ssim_metric) {
+                    const float ssim = (float)J.ssim_sum / std::max(1.0f, J.mask_sum);   /* sad_cuda.cu:2443-2448 */
+                    SadhFitResult* r = J.io.result;
+                    if (r && r->log_ssim && J.n_logs < J.io.log_cap) r->log_ssim[J.n_logs] = ssim;
+                    J.final_ssim = ssim;
+                    J.best_ssim = std::max(J.best_ssim, ssim);
+                }
                 J.n_logs++;
                 J.final_psnr = psnr;
                 J.best = std::max(J.best, psnr);
@@ -542,10 +574,37 @@ struct Engine {
             for (auto& f : fits) {
                 if (f->iter < f->iterations) { keep.push_back(std::move(f)); continue; }
                 HIP_TRY(hipMemcpy(f->io.sites_io, f->d_sites, sizeof(SadSite) * f->cap, hipMemcpyDeviceToHost));
+                SAD_TRY(finish_renders(*f, S));
                 done.push_back(std::move(f));
             }
             fits.swap(keep);
             table_dirty = true;
+        }
+        return SADH_OK;
+    }
+
+    /* a finished fit: the packed-site PSNR and render (training.swift:676-703) and the final render, as
+     * sadh_fit produces them */
+    int finish_renders(FitRun& J, hipStream_t S)
+    {
+        SadhFitResult* r = J.io.result;
+        const size_t img = sizeof(float) * 3 * (size_t)J.np;
+        if (o.packed_psnr) {
+            SAD_TRY(launch_pack_inference(J.d_sites, J.n, J.width, J.height, J.d_packed, J.d_quant, S));
+            SAD_TRY(launch_unpack_inference(J.d_packed, J.d_quant, J.n, J.width, J.height, J.d_dec, S));
+            SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_dec, J.iss, J.n, J.width, J.height, J.cw, J.ch, S));
+            SAD_TRY(launch_sse64(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
+            double sse = 0.0;
+            HIP_TRY(hipMemcpyAsync(&sse, J.d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
+            if (r && r->packed_render) HIP_TRY(hipMemcpyAsync(r->packed_render, J.d_render, img, hipMemcpyDeviceToHost, S));
+            HIP_TRY(hipStreamSynchronize(S));
+            const float mse = (float)sse / (std::max(1.0f, J.mask_sum) * 3.0f);
+            J.packed_psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
+        }
+        if (r && r->final_render) {
+            SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw, J.ch, S));
+            HIP_TRY(hipMemcpyAsync(r->final_render, J.d_render, img, hipMemcpyDeviceToHost, S));
+            HIP_TRY(hipStreamSynchronize(S));
         }
         return SADH_OK;
     }
@@ -556,6 +615,9 @@ struct Engine {
         if (!r) return;
         r->final_psnr = J.final_psnr;
         r->best_psnr = J.best;
+        r->final_ssim = J.final_ssim;
+        r->best_ssim = J.best_ssim;
+        r->packed_psnr = J.packed_psnr;
         r->actual_sites = J.actual;
         r->active_estimate = J.active_est;
         r->active_count = (int)J.active_count;

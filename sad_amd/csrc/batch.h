@@ -39,6 +39,8 @@ struct BatchJob {
     float* tau_tmp2;
     uint32_t* tau_nbr;         /* 8 ids per site: neighbour lists shared by the tau passes */
     double* removal;           /* prune iterations: removal deltas */
+    const float* rendered;     /* SSIM loss: render of the current state, written before the gradient */
+    float ssim_weight;         /* SSIM loss weight (0: L2 only) */
     const uint32_t* h_order;   /* null: no Hilbert probes */
     const uint32_t* h_pos;
     uint32_t n;                /* current site count */

@@ -659,6 +659,8 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.mask = J.mask;
     A.gline64 = J.gacc;
     A.removal64 = J.removal;
+    A.rendered = J.rendered;
+    A.ssim_weight = J.ssim_weight;
     A.gstride = 16u;
     A.iss = J.iss;
     A.inv_scale = J.inv_scale;

@@ -67,7 +67,8 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
     return out
 
 
-def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list | None = None, **over) -> dict:
+def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list | None = None,
+              want_render: bool = False, **over) -> dict:
     """Fit many images in lock step with one launch per kernel per iteration
     (sadh_fit_batch).  targets[i] float32 [H][W][3], sites[i] float32 [N_i][10];
     all jobs share the options.  Returns {"jobs": [per-job dicts as fit()],
@@ -104,14 +105,25 @@ def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list 
         jobs[i].log_psnr = _lib.ptr(logs)
         jobs[i].log_cap = len(logs)
         jobs[i].result = C.pointer(results[i])
-        outs.append(dict(sites=buf, plan=plan, _logs=logs))
+        slogs = np.zeros(len(logs), np.float32)
+        results[i].log_ssim = slogs.ctypes.data_as(C.POINTER(C.c_float))
+        render = np.zeros((h, w, 3), np.float32) if want_render else None
+        if render is not None:
+            results[i].final_render = render.ctypes.data_as(C.POINTER(C.c_float))
+        prender = np.zeros((h, w, 3), np.float32) if (want_render and o.get("packed_psnr")) else None
+        if prender is not None:
+            results[i].packed_render = prender.ctypes.data_as(C.POINTER(C.c_float))
+        keep += [slogs, render, prender]
+        outs.append(dict(sites=buf, plan=plan, _logs=logs, _slogs=slogs, render=render, packed_render=prender))
     br = _lib.BatchResult()
     s = to_struct(o)
     _lib.call("sadh_fit_batch", C.byref(s), jobs, n, C.byref(br))
     for out, res in zip(outs, results):
         out.update(_lib.fit_result_dict(res))
         logs = out.pop("_logs")
+        slogs = out.pop("_slogs")
         out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
+        out["log_ssim"] = slogs[: min(res.n_logs, len(logs))].copy()
     ret = {k: getattr(br, k) for k, _ in _lib.BatchResult._fields_}
     ret["jobs"] = outs
     return ret

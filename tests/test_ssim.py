@@ -10,8 +10,9 @@ CPU tests pin the oracle:
   * the gradient with the SSIM term against the L2 gradient plus the autograd
     backward of grad_output = -w dSSIM/dx_k (0.299, 0.587, 0.114): the site
     gradient is linear in dL/dpred.
-GPU tests: the metric and the loss-term gradient against the oracle, and a fit
-with --ssim-weight and --ssim against the oracle's fit."""
+GPU tests: the metric and the loss-term gradient against the oracle, a fit
+with --ssim-weight and --ssim against the oracle's fit, and the batched engine
+with both (and the packed PSNR) against single fits."""
 import numpy as np
 import pytest
 
@@ -153,3 +154,27 @@ def test_gpu_fit_ssim(orc):
     # the SSIM term changes the fit: the same fit without it ends elsewhere
     plain = trainer.fit(t, s0, dict(opts, ssim_weight=0.0))
     assert abs(plain["final_psnr"] - g["final_psnr"]) > 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_batch_ssim_and_packed_equal_single_fits():
+    """The batched engine with the SSIM loss, the SSIM metric and the packed
+    PSNR: each job ends exactly where its own sadh_fit ends (the fits repeat
+    bit for bit), with the same logged SSIMs, packed PSNR and renders."""
+    from sad_amd import trainer
+    specs = [(128, 96, 41, 700), (90, 140, 42, 500)]
+    tg = [synth.make_target(w, h, sd, n_ellipses=20) for w, h, sd, _ in specs]
+    st = [synth.init_sites_uniform(t, n, sd) for t, (_, _, sd, n) in zip(tg, specs)]
+    opts = config.fit_options(iterations=90, target_bpp=1.5, densify_freq=30, prune_freq=30, densify_start=30,
+                              prune_start=30, max_sites=1500, log_freq=30, ssim_weight=0.5, ssim_metric=1,
+                              packed_psnr=1)
+    b = trainer.fit_batch(tg, st, opts, want_render=True)["jobs"]
+    for i, (t, s0) in enumerate(zip(tg, st)):
+        g = trainer.fit(t, s0, dict(opts, n_sites=s0.shape[0]), want_render=True)
+        print(i, "batch", b[i]["log_ssim"], b[i]["packed_psnr"], "single", g["log_ssim"], g["packed_psnr"])
+        assert np.array_equal(b[i]["sites"].view(np.uint32), g["sites"].view(np.uint32))
+        assert np.array_equal(b[i]["log_psnr"], g["log_psnr"]) and np.array_equal(b[i]["log_ssim"], g["log_ssim"])
+        assert len(g["log_ssim"]) == 4 and 0.0 < g["final_ssim"] <= 1.0
+        assert b[i]["packed_psnr"] == g["packed_psnr"] and b[i]["best_ssim"] == g["best_ssim"]
+        assert np.array_equal(b[i]["render"], g["render"])
+        assert np.array_equal(b[i]["packed_render"], g["packed_render"])
