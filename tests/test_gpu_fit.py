@@ -37,6 +37,22 @@ def test_fit_densify_prune_schedule(orc):
     assert abs(g["final_psnr"] - r["final_psnr"]) <= 0.05
 
 
+def test_fit1024_opening_against_oracle(orc):
+    """The full 1024x1024 / 16,384-site fit with its densify/prune plan, through
+    the first densify and prune (iteration 100): schedule identical, PSNR within
+    0.05 dB at every log.  The oracle's 120 iterations take a few seconds."""
+    import os
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    tgt, sites, over = synth.fit_workload("fit1024")
+    opts = config.fit_options(**dict(over, iterations=120, log_freq=40))
+    g = trainer.fit(tgt, sites, opts)
+    r = orc.fit(tgt, sites, opts)
+    print("gpu", g["log_psnr"], g["actual_sites"], "oracle", r["log_psnr"], r["actual_sites"])
+    assert g["plan"] == r["plan"] and g["n_densify"] == r["n_densify"] == 1 and g["n_prune"] == r["n_prune"] == 1
+    assert (g["actual_sites"], g["active_estimate"]) == (r["actual_sites"], r["active_estimate"])
+    assert np.max(np.abs(np.asarray(g["log_psnr"]) - np.asarray(r["log_psnr"]))) <= 0.05
+
+
 def test_render_sites_pipeline(orc):
     w, h, n = 200, 160, 3000
     sites = synth.init_sites_random(w, h, n, 41)

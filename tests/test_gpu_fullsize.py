@@ -1,0 +1,139 @@
+"""Parity at the full size of the render + single-backward workload: a
+4096x4096 target and 1,048,576 sites (uniform positions, log-normal scales,
+random directions and colours).  Each step runs on the GPU and in the oracle on
+the same inputs; the oracle's OpenMP loops finish each in seconds on the host.
+
+  * Hilbert keys of all sites and their stable 1M-pair radix sort: bit-exact;
+  * JFA seed and one flood pass (step 2048) from the seeded field: bit-exact;
+  * packed candidate sites and two VPT passes (jump 2048 from the flooded
+    field, then jump 1 with Hilbert probes and injection): bit-exact lists;
+  * the softmax render: within 1e-5;
+  * the tiled gradient backward: per-site relative bar;
+  * Adam with the gradient clear: bit-exact sites and state."""
+import os
+
+import numpy as np
+import pytest
+
+from sad_amd import synth
+from tests.tolerance import Rel
+
+pytestmark = pytest.mark.gpu
+
+W = H = 4096
+N = 1 << 20
+BITS = 12
+
+
+@pytest.fixture(scope="module")
+def c3(gpu, orc):
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    target = synth.make_target(W, H, 3, 40)
+    sites = synth.init_sites_random(W, H, N, 3)
+    iss = float(np.float32(1.0) / (np.float32(W) * np.float32(W)))
+    return dict(target=target, sites=sites, iss=iss, gs=gpu.put(sites))
+
+
+def _u32(gpu, t):
+    return gpu.get(t, np.uint32)
+
+
+def test_hilbert_and_sort_1m(gpu, orc, c3):
+    padded = N
+    hp = gpu.zeros((padded, 2), np.uint32)
+    gpu.call("sadh_hilbert_pairs", c3["gs"], hp, N, padded, W, H, BITS)
+    ref = orc.hilbert_pairs(c3["sites"], padded, W, H, BITS)
+    got = _u32(gpu, hp)
+    assert np.array_equal(got, ref)
+    scratch = gpu.zeros((padded, 2), np.uint32)
+    hist = gpu.zeros((256 * (padded // 1024),), np.uint32)
+    gpu.call("sadh_radix_sort_pairs", hp, scratch, hist, hist, padded, 1 << (2 * BITS))
+    order = np.argsort(ref[:, 0], kind="stable")
+    assert np.array_equal(_u32(gpu, hp), ref[order])
+
+
+def test_jfa_seed_and_flood(gpu, orc, c3):
+    a0, a1 = gpu.zeros((W * H, 4), np.uint32), gpu.zeros((W * H, 4), np.uint32)
+    gpu.call("sadh_clear_candidates", a0, a1, W, H)
+    gpu.call("sadh_jfa_seed", a0, c3["gs"], N, W, H, 1)
+    r0, _ = orc.clear_candidates(W, H)
+    r0 = orc.jfa_seed(r0, c3["sites"], W, H)
+    assert np.array_equal(_u32(gpu, a0), r0)
+    out = gpu.zeros((W * H, 4), np.uint32)
+    gpu.call("sadh_jfa_flood", a0, out, c3["gs"], N, 2048, c3["iss"], W, H, 1, W, H)
+    ref = orc.jfa_flood(r0, c3["sites"], 2048, c3["iss"], W, H)
+    assert np.array_equal(_u32(gpu, out), ref)
+
+
+@pytest.fixture(scope="module")
+def field(gpu, orc, c3):
+    """a GPU-built field: clear, seed, the 12 flood passes, then one VPT pass at
+    jump 2048 and one at jump 1 checked against the oracle on the way"""
+    gs = c3["gs"]
+    a0, a1 = gpu.zeros((W * H, 4), np.uint32), gpu.zeros((W * H, 4), np.uint32)
+    b0, b1 = gpu.zeros((W * H, 4), np.uint32), gpu.zeros((W * H, 4), np.uint32)
+    gpu.call("sadh_clear_candidates", a0, a1, W, H)
+    gpu.call("sadh_jfa_seed", a0, gs, N, W, H, 1)
+    step = 2048
+    while step >= 1:
+        gpu.call("sadh_jfa_flood", a0, b0, gs, N, step, c3["iss"], W, H, 1, W, H)
+        a0, b0 = b0, a0
+        step >>= 1
+    pk = gpu.zeros((N, 8), np.uint16)
+    gpu.call("sadh_pack_candidate_sites", gs, pk, N)
+    hp = gpu.zeros((N, 2), np.uint32)
+    gpu.call("sadh_hilbert_pairs", gs, hp, N, N, W, H, BITS)
+    scratch = gpu.zeros((N, 2), np.uint32)
+    hist = gpu.zeros((256 * (N // 1024),), np.uint32)
+    gpu.call("sadh_radix_sort_pairs", hp, scratch, hist, hist, N, 1 << (2 * BITS))
+    order, pos = gpu.zeros((N,), np.uint32), gpu.zeros((N,), np.uint32)
+    gpu.call("sadh_hilbert_order", hp, order, pos, N)
+    checks = []
+    for idx, jump in ((0, 2048), (40, 1)):
+        stp = (jump << 16) | idx
+        gpu.call("sadh_update_candidates", a0, a1, b0, b1, pk, N, stp, c3["iss"], 0, 64.0, 0, 4, order, pos, 1, 32,
+                 1, W, H, W, H)
+        checks.append((idx, _u32(gpu, a0), _u32(gpu, a1), _u32(gpu, b0), _u32(gpu, b1)))
+        a0, b0, a1, b1 = b0, a0, b1, a1
+    return dict(a0=a0, a1=a1, pk=gpu.get(pk, np.uint16), order=_u32(gpu, order), pos=_u32(gpu, pos), checks=checks)
+
+
+def test_pack_and_vpt_passes(orc, c3, field):
+    assert np.array_equal(field["pk"], orc.pack_sites(c3["sites"]))
+    for idx, i0, i1, o0, o1 in field["checks"]:
+        jump = 2048 if idx == 0 else 1
+        r0, r1 = orc.update_candidates(i0, i1, field["pk"], (jump << 16) | idx, c3["iss"], W, H, radius_scale=64.0,
+                                       inject_count=4, hilbert_order=field["order"], hilbert_pos=field["pos"],
+                                       hilbert_probes=1, hilbert_window=32)
+        assert np.array_equal(o0, r0) and np.array_equal(o1, r1)
+        assert np.mean(np.any(o0 != i0, axis=1)) > 0.001          # the pass changed the field
+
+
+def test_render(gpu, orc, c3, field):
+    out = gpu.zeros((H, W, 3))
+    gpu.call("sadh_render_voronoi", field["a0"], field["a1"], out, c3["gs"], c3["iss"], N, W, H, W, H)
+    ref = orc.render(gpu.get(field["a0"], np.uint32), gpu.get(field["a1"], np.uint32), c3["sites"], c3["iss"], W, H)
+    assert np.max(np.abs(gpu.get(out) - ref)) <= 1e-5
+
+
+def test_gradient_and_adam(gpu, orc, c3, field):
+    c0, c1 = gpu.get(field["a0"], np.uint32), gpu.get(field["a1"], np.uint32)
+    gt = gpu.put(c3["target"])
+    gm = gpu.put(np.ones((H, W), np.float32))
+    grads = [gpu.zeros((N,)) for _ in range(10)]
+    rd = gpu.zeros((N,))
+    gpu.call("sadh_compute_gradients_tiled", field["a0"], field["a1"], gt, gt, gm, *grads, c3["gs"], c3["iss"], N, rd,
+             0, 0.0, W, H, W, H)
+    acc, ab, _, _ = orc.gradients(c0, c1, c3["target"], np.ones((H, W), np.float32), c3["sites"], c3["iss"], W, H)
+    got = np.stack([gpu.get(g) for g in grads], 1)
+    assert Rel(got, acc, ab) <= 1.0
+    # Adam + clear from these gradients, against the oracle on the same float32 gradients
+    lrs = (0.025, 0.01, 0.01, 0.01, 0.0025, 0.025)
+    state = gpu.zeros((N, 20))
+    sites = gpu.put(c3["sites"])
+    gpu.call("sadh_adam_update_clear", sites, state, *grads, *lrs, 0.9, 0.999, 1e-8, 1, N, W, H)
+    rs, rst = orc.adam(c3["sites"], np.zeros((N, 20), np.float32), [got[:, j] for j in range(10)], lrs, 0.9, 0.999,
+                       1e-8, 1, W, H)
+    assert np.array_equal(gpu.get(sites).view(np.uint32), rs.view(np.uint32))
+    assert np.array_equal(gpu.get(state).view(np.uint32), rst.view(np.uint32))
+    assert all(np.all(gpu.get(g) == 0) for g in grads)
