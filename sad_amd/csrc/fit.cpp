@@ -225,7 +225,6 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     HIP_TRY(hipStreamCreateWithFlags(&sp.cand, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&sp.cand_done, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&sp.sites_ready, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&sp.lists_done, hipEventDisableTiming));
     hipStream_t S = sp.main;
 
     HIP_TRY(hipMemcpyAsync(d_target, target_h, sizeof(float) * 3 * np, hipMemcpyHostToDevice, S));
@@ -343,21 +342,6 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         const float blend = (float)iter / (float)std::max(1, o.iterations);
         const float lambda = 0.05f * (0.1f + 0.9f * blend);
-        /* The tau passes' neighbour lists depend on positions, activity and the field the tail uses, not on
-         * the gradients: build them on the candidate stream, after this iteration's field update or the
-         * previous iteration's site update, while the gradient runs; after a densify (sites changed on the
-         * main stream just now) build them there. */
-        {
-            const uint32_t* lf0 = pending ? nA0 : cA0;
-            const uint32_t* lf1 = pending ? nA1 : cA1;
-            if (densify) {
-                SAD_TRY(launch_tau_lists(lf0, lf1, d_sites, n, lambda, cw, ch, ds, tau_nbr, S));
-            } else {
-                if (!pending) HIP_TRY(hipStreamWaitEvent(sp.cand, sp.sites_ready, 0));
-                SAD_TRY(launch_tau_lists(lf0, lf1, d_sites, n, lambda, cw, ch, ds, tau_nbr, sp.cand));
-                HIP_TRY(hipEventRecord(sp.lists_done, sp.cand));
-            }
-        }
         const bool prune = p_pct > 0.0f && iter >= eff_ps && iter < eff_pe && iter % std::max(1, o.prune_freq) == 0 &&
                            needs_pairs && needs_prune;
         if (prune) SAD_TRY(launch_clear64(&removal, 1, n, S));
@@ -383,11 +367,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
             HIP_TRY(hipStreamWaitEvent(S, sp.cand_done, 0));
             cA0 = nA0; cA1 = nA1; cB0 = nB0; cB1 = nB1;
         }
-        if (!densify) HIP_TRY(hipStreamWaitEvent(S, sp.lists_done, 0));
         /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
         SAD_TRY(launch_tau_adam_lines64(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites,
                                         d_adam, lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr, true));
+                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr));
 
         if (prune) {
             n_prune++;
@@ -400,7 +383,10 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
                 active_est = std::max(0, active_est - k);
             }
         }
-        HIP_TRY(hipEventRecord(sp.sites_ready, S));
+        /* the next iteration's candidate update (second stream) reads the sites Adam just wrote; an event
+         * record drains the queue for a few microseconds, so record it only before such an iteration */
+        if (o.cand_update_freq > 0 && (iter + 1) % o.cand_update_freq == 0)
+            HIP_TRY(hipEventRecord(sp.sites_ready, S));
 
         if (iter % std::max(1, o.log_freq) == 0 || iter == o.iterations - 1) {
             SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));

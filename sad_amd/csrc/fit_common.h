@@ -89,10 +89,9 @@ struct DevArena {
 
 struct StreamPair {
     hipStream_t main = nullptr, cand = nullptr;
-    hipEvent_t cand_done = nullptr, sites_ready = nullptr, lists_done = nullptr;
+    hipEvent_t cand_done = nullptr, sites_ready = nullptr;
     ~StreamPair()
     {
-        if (lists_done) (void)hipEventDestroy(lists_done);
         if (cand_done) (void)hipEventDestroy(cand_done);
         if (sites_ready) (void)hipEventDestroy(sites_ready);
         if (cand) (void)hipStreamDestroy(cand);
@@ -112,15 +111,14 @@ int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int heigh
                           SadInferenceQuant* quant, hipStream_t stream);
 int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
                             int height, SadSite* out, hipStream_t stream);
-/* nbr_lists (nullable): 8 uint32 per site of scratch for the neighbour lists the passes share;
- * lists_built: the lists are already there (launch_tau_lists), the first pass only reads them */
+/* nbr_lists (nullable): 8 uint32 per site of scratch for the neighbour lists the passes share */
 int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
                         SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
-                        uint32_t, int, int, hipStream_t, uint32_t* nbr_lists = nullptr, bool lists_built = false);
+                        uint32_t, int, int, hipStream_t, uint32_t* nbr_lists = nullptr);
 /* the same tail on double [N][16] lines; requires nbr_lists */
 int launch_tau_adam_lines64(const uint32_t*, const uint32_t*, double*, float*, float*, float*, float, int, int, int,
                             SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
-                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists, bool lists_built);
+                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists);
 /* the fit's double cross-tile sums: gradient lines and removal deltas, site statistics */
 int launch_gradients_lines64(const uint32_t*, const uint32_t*, const float*, const float*, double*, const SadSite*,
                              float, uint32_t, double*, bool, int, int, int, int, hipStream_t,
@@ -132,9 +130,6 @@ int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const doubl
 int launch_densify_pairs(const SadSite*, const double*, const double*, SadPair*, uint32_t, float, float, uint32_t,
                          hipStream_t);
 int launch_prune_pairs(const SadSite*, const double*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
-/* the tau passes' neighbour lists of the n sites on field (c0, c1), for lambda > 0 */
-int launch_tau_lists(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, uint32_t n, float lambda, int cw,
-                     int ch, int ds, uint32_t* nbr_lists, hipStream_t stream);
 
 /* batched fit (one launch over every job) */
 int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);

@@ -235,35 +235,6 @@ __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restric
     gout[i] = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
-/* the list alone (tau_site_list without the pass): written before the
- * gradients exist, off their critical path */
-__device__ __forceinline__ void tau_list_only(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                              const SadSite* __restrict__ sites, uint32_t n, float lambda, int width,
-                                              int height, float inv_ds, uint4* __restrict__ nbr)
-{
-    const float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
-    if (lambda <= 0.0f || pos.x < 0.0f) {
-        nbr[2u * i] = make_uint4(NBR_COPY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
-        nbr[2u * i + 1u] = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
-        return;
-    }
-    const int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
-    const int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
-    const uint4 a = c0[py * width + px], b = c1[py * width + px];
-    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t keep[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t nb = ids[k];
-        bool ok = nb < n && nb != i;
-#pragma unroll
-        for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
-        keep[k] = (ok && sites[nb].pos_x >= 0.0f) ? nb : SAD_EMPTY;
-    }
-    nbr[2u * i] = make_uint4(keep[0], keep[1], keep[2], keep[3]);
-    nbr[2u * i + 1u] = make_uint4(keep[4], keep[5], keep[6], keep[7]);
-}
-
 /* one pass from the list: the value tau_site computes (graw and gin hold site i at [i * stride]) */
 template <typename TR, typename TI>
 __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const TR* __restrict__ graw,
@@ -282,26 +253,6 @@ __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restri
         }
     }
     return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
-}
-
-__global__ __launch_bounds__(256) void k_tau_lists(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                   const SadSite* __restrict__ sites, uint32_t n, float lambda,
-                                                   int width, int height, float inv_ds, uint4* __restrict__ nbr)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) tau_list_only(i, c0, c1, sites, n, lambda, width, height, inv_ds, nbr);
-}
-
-/* first pass from built lists, reading the log_tau column of the gradient lines; saves raw */
-template <typename T>
-__global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __restrict__ nbr,
-                                                              const T* __restrict__ g2, float* __restrict__ gout,
-                                                              float* __restrict__ raw_copy, uint32_t n, float lambda)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    raw_copy[i] = (float)g2[i * 16u];
-    gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
 }
 
 template <typename T>
@@ -979,23 +930,19 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
 }
 
 /* The list-driven tail on [N][16] gradient lines of type T: pass 1 reads the
- * strided log_tau column (raw saved) and, unless lists_built, writes the
- * neighbour lists; passes 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam,
- * which reads and clears each site's line. */
+ * strided log_tau column (raw saved) and writes the neighbour lists; passes
+ * 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam, which reads and clears
+ * each site's line. */
 template <typename T>
 static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw, float* tmp, float* tmp2,
                            float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
-                           uint32_t* nbr_lists, bool lists_built, const AdamArgs& A, hipStream_t stream)
+                           uint32_t* nbr_lists, const AdamArgs& A, hipStream_t stream)
 {
     const uint32_t n = A.n;
     const dim3 grid(div_up(n, 256u));
     uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
     const T* g2 = gacc + 2;
-    if (lists_built)
-        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(256), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
-                           lambda);
-    else
-        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
+    hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
                            raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
     hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp, tmp2, n, lambda);
@@ -1013,7 +960,7 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
                         float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                         float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                         float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                        hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
+                        hipStream_t stream, uint32_t* nbr_lists)
 {
     if (n == 0) return SADH_OK;
     const float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
@@ -1028,7 +975,7 @@ int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, flo
     A.gstride = 16u;
     if (nbr_lists) {
         tau_adam_lines(u0, u1, gacc, raw, tmp, tmp2, lambda, cand_width, cand_height, inv_ds, sites, nbr_lists,
-                       lists_built, A, stream);
+                       A, stream);
         return launch_status("tau_diffuse_adam(aos, lists)");
     }
     hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
@@ -1048,7 +995,7 @@ int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc
                             float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                             float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                             float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                            hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
+                            hipStream_t stream, uint32_t* nbr_lists)
 {
     if (n == 0) return SADH_OK;
     if (!nbr_lists) return fail(SADH_ERR_INVALID, "tau_diffuse_adam(lines64): neighbour lists are required");
@@ -1058,19 +1005,9 @@ int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc
               height);
     A.g64 = gacc;
     tau_adam_lines(reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), gacc, raw, tmp, tmp2,
-                   lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, lists_built, A,
+                   lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, A,
                    stream);
     return launch_status("tau_diffuse_adam(lines64)");
-}
-
-int launch_tau_lists(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, uint32_t n, float lambda, int cw,
-                     int ch, int ds, uint32_t* nbr_lists, hipStream_t stream)
-{
-    if (n == 0) return SADH_OK;
-    hipLaunchKernelGGL(k_tau_lists, dim3(div_up(n, 256u)), dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
-                       reinterpret_cast<const uint4*>(c1), sites, n, lambda, cw, ch, 1.0f / fmaxf((float)ds, 1.0f),
-                       reinterpret_cast<uint4*>(nbr_lists));
-    return launch_status("tau_lists");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
