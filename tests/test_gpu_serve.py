@@ -1,7 +1,7 @@
 """Continuous-batching server: requests that arrive while others run are
 admitted, overlap on the device, and each result equals the same request run
-alone (fits: identical schedule and final PSNR within 0.05 dB, also against
-the oracle; renders: the same image bit for bit)."""
+alone, bit for bit (fits: sites and PSNR log; renders: the image).  Fits are
+also held to the oracle's schedule and final PSNR within 0.05 dB."""
 import numpy as np
 import pytest
 
@@ -49,7 +49,8 @@ def test_mixed_stream_matches_alone(orc):
         print(i, "mix", g["log_psnr"], "alone", alone["log_psnr"], "oracle", r["log_psnr"])
         key = ("n_densify", "n_prune", "actual_sites", "active_estimate", "n_cand_updates")
         assert tuple(g[k] for k in key) == tuple(alone[k] for k in key) == tuple(r[k] for k in key)
-        assert abs(g["final_psnr"] - alone["final_psnr"]) <= 0.05
+        assert np.array_equal(g["sites"].view(np.uint32), alone["sites"].view(np.uint32))
+        assert np.array_equal(g["log_psnr"], alone["log_psnr"])
         assert abs(g["final_psnr"] - r["final_psnr"]) <= 0.05
     for i, (s, w, h) in enumerate(renders):
         alone = trainer.render_sites(s, w, h, cand_passes=16)
@@ -69,7 +70,7 @@ def test_memory_budget_admits_in_turn():
         ra, rb = srv.result(ia), srv.result(ib)
     assert rb["admit_s"] >= ra["done_s"]
     for (t, s0), r in ((a, ra), (b, rb)):
-        assert abs(r["final_psnr"] - trainer.fit(t, s0, opts)["final_psnr"]) <= 0.05
+        assert np.array_equal(r["sites"].view(np.uint32), trainer.fit(t, s0, opts)["sites"].view(np.uint32))
 
 
 def test_request_larger_than_budget_is_refused():
