@@ -48,6 +48,24 @@ int sadh_render_voronoi(const uint32_t* cand0, const uint32_t* cand1, float* out
                         int width, int height, int cand_width, int cand_height,
                         sadh_stream_t stream);
 
+/* renderVoronoiColoring, backends/metal/shaders/encoders/render_encoder.metal:73-134
+ * (encodeColoring, backends/metal/sources/encoders/render_encoder.swift:52-71): the
+ * render with each site's colour replaced by a hash of its id (the _cells.png
+ * picture of --render-cells and --include-debug-mask). */
+int sadh_render_cells(const uint32_t* cand0, const uint32_t* cand1, float* output,
+                      const SadSite* sites, float inv_scale_sq, uint32_t site_count,
+                      int width, int height, int cand_width, int cand_height,
+                      sadh_stream_t stream);
+
+/* renderCentroidsTauHeatmap, backends/metal/shaders/encoders/render_encoder.metal:136-190
+ * (encodeTauHeatmap, render_encoder.swift:73-93): dots of radius dot_radius at the
+ * candidate sites, blue -> white -> red over min_tau -> mean_tau -> max_tau
+ * (the _tau_heatmap.png picture of --include-debug-mask). */
+int sadh_render_tau_heatmap(const uint32_t* cand0, const uint32_t* cand1, float* output,
+                            const SadSite* sites, uint32_t site_count, float min_tau,
+                            float mean_tau, float max_tau, float dot_radius, int width, int height,
+                            int cand_width, int cand_height, sadh_stream_t stream);
+
 /* render_sad_backward_kernel launch, backends/pytorch/torch_ext/cuda/sad_bindings.cu:100-207,733-746.
  * grad_output [H][W][3]; grad_sites [N][10] (Site field order), accumulated into (caller zeroes). */
 int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const float* grad_output,
@@ -323,6 +341,10 @@ typedef struct SadhFitResult {
     float* log_ssim;                  /* in: optional host array of log_cap entries; out: the logged SSIMs */
     float packed_psnr;                /* packed_psnr: PSNR of the quantised inference sites */
     float* packed_render;             /* in: optional host [H][W][3]; out: that render (packed_psnr) */
+    float* cells_render;              /* in: optional host [H][W][3]; out: sadh_render_cells of the final sites
+                                         (--include-debug-mask, training.swift:636-671) */
+    float* tau_heatmap;               /* in: optional host [H][W][3]; out: sadh_render_tau_heatmap of the final
+                                         sites over their tau range, dot radius 2 (training.swift:641-652) */
 } SadhFitResult;
 
 /* Resolve densify/prune percentiles and buffer capacity as trainVoronoi does
@@ -404,12 +426,14 @@ int sadh_server_destroy(sadh_server_t server);
 
 /* Render mode of renderVoronoiFromSites, sad_cuda.cu:1678-1858: JFA (jfa_rounds
  * rounds of clear + seed + log2 floods) and `cand_passes` VPT passes, then the
- * softmax render.  sites and output are host arrays; timings in ms (nullable). */
+ * softmax render.  sites and output are host arrays; timings in ms (nullable);
+ * cells_output (nullable host [H][W][3]) also receives sadh_render_cells of the
+ * same candidates (--render-cells, backends/metal/sources/rendering/renderer.swift:213-218). */
 int sadh_render_sites(const SadSite* sites, uint32_t site_count, int width, int height,
                       int cand_passes, int use_jfa, int jfa_rounds, int cand_downscale,
                       float radius_scale, uint32_t radius_probes, uint32_t inject_count,
                       uint32_t hilbert_window, uint32_t hilbert_probes, float* output,
-                      float* cand_ms, float* render_ms);
+                      float* cand_ms, float* render_ms, float* cells_output);
 
 /* renderPackedInference, backends/metal/sources/rendering/packed_renderer.swift:15-137
  * (render mode --render-packed): the render job of sadh_render_sites on
@@ -418,7 +442,8 @@ int sadh_render_packed_sites(const SadInferenceSite* packed, const SadInferenceQ
                              uint32_t site_count, int width, int height, int cand_passes, int use_jfa,
                              int jfa_rounds, int cand_downscale, float radius_scale,
                              uint32_t radius_probes, uint32_t inject_count, uint32_t hilbert_window,
-                             uint32_t hilbert_probes, float* output, float* cand_ms, float* render_ms);
+                             uint32_t hilbert_probes, float* output, float* cand_ms, float* render_ms,
+                             float* cells_output);
 
 #ifdef __cplusplus
 }

@@ -120,6 +120,31 @@ def render(cand0, cand1, sites, iss, width, height, cand_w=None, cand_h=None):
     return out
 
 
+def render_cells(cand0, cand1, sites, iss, width, height, cand_w=None, cand_h=None):
+    """render with each site's colour replaced by sad_hash_color(id)"""
+    cand_w = cand_w or width; cand_h = cand_h or height
+    c0, c1, s = _u32(cand0), _u32(cand1), _f32(sites)
+    out = np.zeros((height, width, 3), np.float32)
+    lib().orc_render_cells(_p(c0), _p(c1), _p(out), _p(s), F32(iss), U32(s.shape[0]), I32(width), I32(height),
+                           I32(cand_w), I32(cand_h))
+    return out
+
+
+def tau_heatmap(cand0, cand1, sites, tmin, tmean, tmax, dot, width, height, cand_w=None, cand_h=None):
+    c0, c1, s = _u32(cand0), _u32(cand1), _f32(sites)
+    out = np.zeros((height, width, 3), np.float32)
+    lib().orc_tau_heatmap(_p(c0), _p(c1), _p(out), _p(s), U32(s.shape[0]), F32(tmin), F32(tmean), F32(tmax),
+                          F32(dot), I32(width), I32(height), I32(cand_w or width), I32(cand_h or height))
+    return out
+
+
+def hash_colors(ids):
+    ids = np.ascontiguousarray(ids, np.uint32)
+    out = np.zeros((ids.size, 3), np.float32)
+    lib().orc_hash_colors(_p(ids), U32(ids.size), _p(out))
+    return out
+
+
 def init_candidates(n, seed, per_pixel, width, height):
     c0 = np.zeros((width * height, 4), np.uint32); c1 = np.zeros_like(c0)
     lib().orc_init_candidates(_p(c0), _p(c1), U32(n), U32(seed), I32(int(per_pixel)), I32(width), I32(height))

@@ -84,9 +84,10 @@ static void orc_load_ids(const uint32_t* c0, const uint32_t* c1, int cell, uint3
 
 /* renderVoronoiKernel, render.cu:5-92.  The colour blend additionally skips
  * NaN/Inf weights (render.cu:83), which the gradient kernel does not. */
-ORC_API void orc_render(const uint32_t* cand0, const uint32_t* cand1, float* out,
-                        const SadSite* sites, float iss, uint32_t n,
-                        int width, int height, int cand_w, int cand_h)
+/* cells != 0: the site colours replaced by sad_hash_color of the id
+ * (renderVoronoiColoring, shaders/encoders/render_encoder.metal:73-134) */
+static void render_impl(const uint32_t* cand0, const uint32_t* cand1, float* out, const SadSite* sites, float iss,
+                        uint32_t n, int width, int height, int cand_w, int cand_h, int cells)
 {
     #pragma omp parallel for schedule(static)
     for (int y = 0; y < height; ++y) {
@@ -115,12 +116,65 @@ ORC_API void orc_render(const uint32_t* cand0, const uint32_t* cand1, float* out
                 float wi = w[i] * inv;
                 uint32_t id = ids[i];
                 if (!isnan(wi) && !isinf(wi) && id < n && sites[id].pos_x >= 0.0f) {
-                    c[0] += wi * sites[id].col_r;
-                    c[1] += wi * sites[id].col_g;
-                    c[2] += wi * sites[id].col_b;
+                    float col[3] = {sites[id].col_r, sites[id].col_g, sites[id].col_b};
+                    if (cells) sad_hash_color(id, col);
+                    c[0] += wi * col[0];
+                    c[1] += wi * col[1];
+                    c[2] += wi * col[2];
                 }
             }
             out[3 * p] = c[0]; out[3 * p + 1] = c[1]; out[3 * p + 2] = c[2];
+        }
+    }
+}
+
+ORC_API void orc_render(const uint32_t* cand0, const uint32_t* cand1, float* out,
+                        const SadSite* sites, float iss, uint32_t n,
+                        int width, int height, int cand_w, int cand_h)
+{
+    render_impl(cand0, cand1, out, sites, iss, n, width, height, cand_w, cand_h, 0);
+}
+
+ORC_API void orc_render_cells(const uint32_t* cand0, const uint32_t* cand1, float* out,
+                              const SadSite* sites, float iss, uint32_t n,
+                              int width, int height, int cand_w, int cand_h)
+{
+    render_impl(cand0, cand1, out, sites, iss, n, width, height, cand_w, cand_h, 1);
+}
+
+/* hashColor (shaders/sad_common.metal:542-565) of each id */
+ORC_API void orc_hash_colors(const uint32_t* ids, uint32_t n, float* rgb)
+{
+    for (uint32_t i = 0; i < n; ++i) sad_hash_color(ids[i], rgb + 3 * (size_t)i);
+}
+
+/* renderCentroidsTauHeatmap, shaders/encoders/render_encoder.metal:136-190 */
+ORC_API void orc_tau_heatmap(const uint32_t* cand0, const uint32_t* cand1, float* out, const SadSite* sites,
+                             uint32_t n, float tmin, float tmean, float tmax, float dot, int width, int height,
+                             int cand_w, int cand_h)
+{
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            size_t p = (size_t)y * width + x;
+            uint32_t ids[SAD_K];
+            orc_load_ids(cand0, cand1, sad_cand_index(x, y, width, height, cand_w, cand_h), ids);
+            float acc[3] = {0.0f, 0.0f, 0.0f}, acc_a = 0.0f;
+            for (int i = 0; i < SAD_K; ++i) {
+                uint32_t id = ids[i];
+                if (id >= n || sites[id].pos_x < 0.0f) continue;
+                const SadSite* s = &sites[id];
+                float a = sad_dot_alpha((float)x - s->pos_x, (float)y - s->pos_y, dot);
+                if (a <= 0.0f) continue;
+                float col[3];
+                sad_heat_color(sad_expf(s->log_tau), tmin, tmean, tmax, col);
+                float blend = a * (1.0f - acc_a);
+                acc[0] += col[0] * blend;
+                acc[1] += col[1] * blend;
+                acc[2] += col[2] * blend;
+                acc_a += blend;
+            }
+            for (int c = 0; c < 3; ++c) out[3 * p + c] = acc[c] + 0.05f * (1.0f - acc_a);
         }
     }
 }

@@ -48,13 +48,15 @@ class FitResult(C.Structure):
         ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
         ("grad_bytes", C.c_double), ("grad_ms_trace", C.POINTER(F32)), ("grad_ms_trace_cap", I32),
         ("final_render", C.POINTER(F32)), ("final_ssim", F32), ("best_ssim", F32), ("log_ssim", C.POINTER(F32)),
-        ("packed_psnr", F32), ("packed_render", C.POINTER(F32)),
+        ("packed_psnr", F32), ("packed_render", C.POINTER(F32)), ("cells_render", C.POINTER(F32)),
+        ("tau_heatmap", C.POINTER(F32)),
     ]
 
 
 def fit_result_dict(res: "FitResult") -> dict:
     """SadhFitResult fields as a dict, without its host-pointer fields."""
-    skip = ("grad_ms_trace", "grad_ms_trace_cap", "final_render", "log_ssim", "packed_render")
+    skip = ("grad_ms_trace", "grad_ms_trace_cap", "final_render", "log_ssim", "packed_render", "cells_render",
+            "tau_heatmap")
     return {k: getattr(res, k) for k, _ in FitResult._fields_ if k not in skip}
 
 
@@ -88,6 +90,8 @@ class ServerEvent(C.Structure):
 # name -> argument types (all return int)
 SIGNATURES = {
     "sadh_render_voronoi": [P, P, P, P, F32, U32, I32, I32, I32, I32, P],
+    "sadh_render_cells": [P, P, P, P, F32, U32, I32, I32, I32, I32, P],
+    "sadh_render_tau_heatmap": [P, P, P, P, U32, F32, F32, F32, F32, I32, I32, I32, I32, P],
     "sadh_render_backward": [P, P, P, P, P, F32, U32, I32, I32, I32, I32, P],
     "sadh_init_candidates": [P, P, U32, U32, I32, I32, I32, P],
     "sadh_clear_candidates": [P, P, I32, I32, P],
@@ -119,7 +123,7 @@ SIGNATURES = {
     "sadh_pack_inference_sites": [P, U32, I32, I32, P, P, P],
     "sadh_unpack_inference_sites": [P, P, U32, I32, I32, P, P],
     "sadh_render_packed_sites": [P, P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
-                                 C.POINTER(F32)],
+                                 C.POINTER(F32), P],
     "sadh_init_gradient_weighted": [P, U32, P, P, P, F32, U32, F32, F32, I32, I32, P],
     "sadh_fit_plan": [C.POINTER(FitOptions), I32, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32),
                       C.POINTER(I32)],
@@ -132,7 +136,7 @@ SIGNATURES = {
     "sadh_server_poll": [P, C.POINTER(ServerEvent), I32, C.POINTER(I32)],
     "sadh_server_destroy": [P],
     "sadh_render_sites": [P, U32, I32, I32, I32, I32, I32, I32, F32, U32, U32, U32, U32, P, C.POINTER(F32),
-                          C.POINTER(F32)],
+                          C.POINTER(F32), P],
 }
 
 _lib = None

@@ -56,10 +56,12 @@ Training options:
   --cand-freq N  --cand-passes N  --cand-downscale N  --cand-radius-scale F  --cand-radius-probes N
   --cand-inject N  --cand-hilbert-window N  --cand-hilbert-probes N  --max-sites N  --lr F  --log-freq N
   --ssim (log SSIM)  --ssim-weight F (loss L2 - F * SSIM)  --packed-psnr (re-render 16-byte sites, PSNR)
+  --include-debug-mask (also write <image>_cells.png and <image>_tau_heatmap.png)
 Render options:
   --out <path>  --width N  --height N  --render-cand-passes N  --render-no-jfa  --render-jfa-rounds N
   --render-cand-downscale N  --render-hilbert-window N  --render-hilbert-probes N  --render-cand-inject N
   --render-target <path>  --render-packed (render the 16-byte quantised sites)
+  --render-cells (also write <sites>_cells.png, hashed cell colours)
 """
 
 
@@ -79,9 +81,9 @@ def parse_args(argv: list, cfg: dict) -> dict:
     if cfg["DEFAULT_TARGET_BPP"] <= 0:
         o["target_bpp"] = 0.0
     run = dict(mode="train", opts=o, target=None, mask=None, init_sites=None, out_dir="results",
-               max_dim=int(cfg["MAX_DIM"]),
+               max_dim=int(cfg["MAX_DIM"]), debug_images=False,
                render=dict(sites=None, out=None, target=None, width=0, height=0, cand_passes=16, use_jfa=True,
-                           packed=False,
+                           packed=False, cells=False,
                            jfa_rounds=1, cand_downscale=int(cfg["CAND_DOWNSCALE"]),
                            radius_scale=float(cfg["CAND_RADIUS_SCALE"]),
                            radius_probes=int(cfg["CAND_RADIUS_PROBES"]), inject_count=int(cfg["CAND_INJECT_COUNT"]),
@@ -148,6 +150,10 @@ def parse_args(argv: list, cfg: dict) -> dict:
                     r["packed"] = True
                 elif a == "--packed-psnr":
                     o["packed_psnr"] = 1
+                elif a in ("--render-cells", "--render-voronoi"):
+                    r["cells"] = True
+                elif a in ("--include-debug-mask", "--include-debug-masks"):
+                    run["debug_images"] = True
                 else:
                     raise UsageError(f"Unknown flag: {a}")
             elif run["mode"] == "train":
@@ -205,7 +211,7 @@ def train(run: dict, out=None) -> dict:
           f" | prune={'on %.3f' % plan['prune_percentile'] if plan['prune_percentile'] > 0 else 'off'}"
           f" | cand=freq {o['cand_update_freq']}, passes {o['cand_update_passes']}, "
           f"downscale {o['cand_downscale']}x", file=out)
-    res = trainer.fit(target, init, o, mask=mask, want_render=True)
+    res = trainer.fit(target, init, o, mask=mask, want_render=True, debug_images=run["debug_images"])
     logs = list(res["log_psnr"])
     its = [k for k in range(o["iterations"]) if k % max(1, o["log_freq"]) == 0 or k == o["iterations"] - 1]
     ssims = list(res["log_ssim"]) if o["ssim_metric"] else [None] * len(logs)
@@ -226,6 +232,12 @@ def train(run: dict, out=None) -> dict:
         sites_io.save_image(packed_path, res["packed_render"])
         print(f"Saved: {packed_path}", file=out)
         print(f"Packed inference PSNR: {res['packed_psnr']:.2f} dB", file=out)
+    if run["debug_images"]:
+        # training.swift:665-671
+        for suffix, img in (("_cells.png", res["cells"]), ("_tau_heatmap.png", res["tau_heatmap"])):
+            path = os.path.join(run["out_dir"], stem + suffix)
+            sites_io.save_image(path, img)
+            print(f"Saved: {path}", file=out)
     print(f"Final PSNR: {res['final_psnr']} dB (best {res['best_psnr']})", file=out)
     if o["ssim_metric"]:
         print(f"Final SSIM: {res['final_ssim']} (best {res['best_ssim']})", file=out)
@@ -247,7 +259,8 @@ def render(run: dict, out=None) -> dict:
     print(f"Render size: {w}x{h}", file=out)
     kw = dict(cand_passes=max(0, r["cand_passes"]), use_jfa=r["use_jfa"], jfa_rounds=r["jfa_rounds"],
               cand_downscale=r["cand_downscale"], radius_scale=r["radius_scale"], radius_probes=r["radius_probes"],
-              inject_count=r["inject_count"], hilbert_window=r["hilbert_window"], hilbert_probes=r["hilbert_probes"])
+              inject_count=r["inject_count"], hilbert_window=r["hilbert_window"], hilbert_probes=r["hilbert_probes"],
+              cells=r["cells"])
     if r["packed"]:
         packed, quant = trainer.pack_inference(sites, w, h)
         job = trainer.render_packed_sites(packed, quant, w, h, **kw)
@@ -267,7 +280,13 @@ def render(run: dict, out=None) -> dict:
     path = r["out"] or (os.path.splitext(r["sites"])[0] + "_render.png")
     sites_io.save_image(path, job["image"])
     print(f"Saved: {path}", file=out)
-    return dict(image=job["image"], path=path, psnr=psnr)
+    cells_path = None
+    if r["cells"]:
+        # cli.swift:833-836: <sites path without extension>_cells.png
+        cells_path = os.path.splitext(r["sites"])[0] + "_cells.png"
+        sites_io.save_image(cells_path, job["cells"])
+        print(f"Saved: {cells_path}", file=out)
+    return dict(image=job["image"], path=path, psnr=psnr, cells=job["cells"], cells_path=cells_path)
 
 
 def main(argv: list | None = None) -> int:

@@ -606,7 +606,9 @@ struct Engine {
             HIP_TRY(hipMemcpyAsync(r->final_render, J.d_render, img, hipMemcpyDeviceToHost, S));
             HIP_TRY(hipStreamSynchronize(S));
         }
-        return SADH_OK;
+        /* the host copy of the sites is in sites_io already */
+        return debug_renders(J.c0(), J.c1(), J.d_sites, J.io.sites_io, J.n, J.iss, J.width, J.height, J.cw, J.ch,
+                             J.d_render, r, S);
     }
 
     void fill_result(const FitRun& J, double secs) const

@@ -453,6 +453,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         HIP_TRY(hipMemcpyAsync(res->final_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
         HIP_TRY(hipStreamSynchronize(S));
     }
+    SAD_TRY(debug_renders(cA0, cA1, d_sites, host_sites.data(), n, iss, width, height, cw, ch, d_render, res, S));
     if (res) {
         res->final_psnr = final_psnr;
         res->best_psnr = best;
@@ -477,11 +478,52 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     return SADH_OK;
 }
 
+void tau_range(const SadSite* sites, uint32_t n, float* tmin, float* tmean, float* tmax)
+{
+    float mn = INFINITY, mx = -INFINITY, sum = 0.0f;
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!(sites[i].pos_x >= 0.0f)) continue;
+        const float tau = sad_expf(sites[i].log_tau);
+        mn = std::min(mn, tau);
+        mx = std::max(mx, tau);
+        sum += tau;
+        count++;
+    }
+    if (count == 0) mn = mx = 0.0f;
+    *tmin = mn;
+    *tmax = mx;
+    *tmean = count > 0 ? sum / (float)count : 0.0f;
+}
+
+int debug_renders(const uint32_t* c0, const uint32_t* c1, const SadSite* d_sites, const SadSite* host_sites,
+                  uint32_t n, float iss, int width, int height, int cw, int ch, float* d_scratch, SadhFitResult* r,
+                  hipStream_t stream)
+{
+    if (!r || !(r->cells_render || r->tau_heatmap)) return SADH_OK;
+    const size_t img = sizeof(float) * 3 * (size_t)width * height;
+    if (r->cells_render) {
+        SAD_TRY(launch_render_cells(c0, c1, d_scratch, d_sites, iss, n, width, height, cw, ch, stream));
+        HIP_TRY(hipMemcpyAsync(r->cells_render, d_scratch, img, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    if (r->tau_heatmap) {
+        float tmin, tmean, tmax;
+        tau_range(host_sites, n, &tmin, &tmean, &tmax);
+        SAD_TRY(launch_tau_heatmap(c0, c1, d_scratch, d_sites, n, tmin, tmean, tmax, 2.0f, width, height, cw, ch,
+                                   stream));
+        HIP_TRY(hipMemcpyAsync(r->tau_heatmap, d_scratch, img, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    return SADH_OK;
+}
+
 /* sites_h, or packed_h + quant_h (quantised inference sites, decoded on the device first) */
 static int render_impl(const SadSite* sites_h, const SadInferenceSite* packed_h, const SadInferenceQuant* quant_h,
                        uint32_t n, int width, int height, int cand_passes, int use_jfa, int jfa_rounds,
                        int cand_downscale, float radius_scale, uint32_t radius_probes, uint32_t inject_count,
-                       uint32_t hil_window, uint32_t hil_probes, float* out_h, float* cand_ms, float* render_ms)
+                       uint32_t hil_window, uint32_t hil_probes, float* out_h, float* cand_ms, float* render_ms,
+                       float* cells_h)
 {
     if (!(sites_h || (packed_h && quant_h)) || !out_h || n == 0 || width <= 0 || height <= 0)
         return fail(SADH_ERR_INVALID, "render_sites: bad arguments");
@@ -587,6 +629,12 @@ static int render_impl(const SadSite* sites_h, const SadInferenceSite* packed_h,
     HIP_TRY(hipEventRecord(e2, S));
     HIP_TRY(hipMemcpyAsync(out_h, d_out, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
     HIP_TRY(hipStreamSynchronize(S));
+    if (cells_h) {
+        /* --render-cells (renderer.swift:213-218): the same candidates, hashed colours */
+        SAD_TRY(launch_render_cells(a0, a1, d_out, d_sites, iss, n, width, height, cw, ch, S));
+        HIP_TRY(hipMemcpyAsync(cells_h, d_out, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
+        HIP_TRY(hipStreamSynchronize(S));
+    }
     float ms = 0.0f;
     if (cand_ms) { HIP_TRY(hipEventElapsedTime(&ms, e0, e1)); *cand_ms = ms; }
     if (render_ms) { HIP_TRY(hipEventElapsedTime(&ms, e1, e2)); *render_ms = ms; }
@@ -616,23 +664,23 @@ int sadh_fit(const SadhFitOptions* opts, const float* target, const float* mask,
 int sadh_render_sites(const SadSite* sites, uint32_t site_count, int width, int height, int cand_passes, int use_jfa,
                       int jfa_rounds, int cand_downscale, float radius_scale, uint32_t radius_probes,
                       uint32_t inject_count, uint32_t hilbert_window, uint32_t hilbert_probes, float* output,
-                      float* cand_ms, float* render_ms)
+                      float* cand_ms, float* render_ms, float* cells_output)
 {
     return render_impl(sites, nullptr, nullptr, site_count, width, height, cand_passes, use_jfa, jfa_rounds,
                        cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
-                       output, cand_ms, render_ms);
+                       output, cand_ms, render_ms, cells_output);
 }
 
 int sadh_render_packed_sites(const SadInferenceSite* packed, const SadInferenceQuant* quant, uint32_t site_count,
                              int width, int height, int cand_passes, int use_jfa, int jfa_rounds, int cand_downscale,
                              float radius_scale, uint32_t radius_probes, uint32_t inject_count,
                              uint32_t hilbert_window, uint32_t hilbert_probes, float* output, float* cand_ms,
-                             float* render_ms)
+                             float* render_ms, float* cells_output)
 {
     if (!packed || !quant) return fail(SADH_ERR_INVALID, "render_packed_sites: null packed sites or quant");
     return render_impl(nullptr, packed, quant, site_count, width, height, cand_passes, use_jfa, jfa_rounds,
                        cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
-                       output, cand_ms, render_ms);
+                       output, cand_ms, render_ms, cells_output);
 }
 
 }  // extern "C"

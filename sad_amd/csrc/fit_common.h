@@ -16,6 +16,17 @@
 
 namespace sadh {
 
+int launch_render_cells(const uint32_t*, const uint32_t*, float*, const SadSite*, float, uint32_t, int, int, int, int,
+                        hipStream_t);
+int launch_tau_heatmap(const uint32_t*, const uint32_t*, float*, const SadSite*, uint32_t, float, float, float, float,
+                       int, int, int, int, hipStream_t);
+/* exportActiveSites, training_report.swift:21-55: min, mean and max of exp(log_tau) over the active sites
+ * (host, index order; all zero when none is active) */
+void tau_range(const SadSite* sites, uint32_t n, float* tmin, float* tmean, float* tmax);
+/* the debug pictures of a finished fit into r->cells_render and r->tau_heatmap (each optional) */
+int debug_renders(const uint32_t* c0, const uint32_t* c1, const SadSite* d_sites, const SadSite* host_sites,
+                  uint32_t n, float iss, int width, int height, int cw, int ch, float* d_scratch, SadhFitResult* r,
+                  hipStream_t stream);
 int launch_render(const uint32_t*, const uint32_t*, float*, const SadSite*, float, uint32_t, int, int, int, int,
                   hipStream_t);
 int launch_gradients(const uint32_t*, const uint32_t*, const float*, const float*, float* const[10], const SadSite*,

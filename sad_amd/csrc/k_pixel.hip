@@ -191,6 +191,7 @@ struct PixelArgs {
     float ssim_weight;         /* GRAD: weight of the -SSIM loss term */
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3] */
+    uint32_t cells;            /* RENDER: 1 = each site's colour is sad_hash_color of its id (cell picture) */
     float* g[10];              /* DST_SOA: value j of site i accumulates at g[j] + i * gstride */
     float* gline;              /* DST_LINES: value j of site i accumulates at gline + i * gstride + j */
     double* gline64;           /* DST_LINES64: value j of site i at gline64 + i * 16 + j */
@@ -376,6 +377,11 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         if (MODE == MODE_RENDER && (isnan(wc) || isinf(wc))) wc = 0.0f;
         if (valid[k]) {
             float4 c = (slots[k] < (uint16_t)H) ? L.p2[slots[k]] : site_terms(A.sites, ids[k], A.inv_scale).p2;
+            if (MODE == MODE_RENDER && A.cells) {
+                float hc[3];
+                sad_hash_color(ids[k], hc);
+                c = make_float4(hc[0], hc[1], hc[2], 0.0f);
+            }
             pr += wc * c.x;
             pgr += wc * c.y;
             pb += wc * c.z;
@@ -700,6 +706,62 @@ int launch_render(const uint32_t* cand0, const uint32_t* cand1, float* out, cons
     return launch_pixel<MODE_RENDER, false>(A, stream, "render_voronoi");
 }
 
+/* renderVoronoiColoring, shaders/encoders/render_encoder.metal:73-134: the render with hashed id colours */
+int launch_render_cells(const uint32_t* cand0, const uint32_t* cand1, float* out, const SadSite* sites, float iss,
+                        uint32_t n, int width, int height, int cw, int ch, hipStream_t stream)
+{
+    PixelArgs A = {};
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.out = out; A.iss = iss; A.n = n; A.cells = 1u;
+    A.width = width; A.height = height; A.cw = cw; A.ch = ch;
+    return launch_pixel<MODE_RENDER, false>(A, stream, "render_cells");
+}
+
+/* renderCentroidsTauHeatmap, shaders/encoders/render_encoder.metal:136-190: one thread per pixel, tau dots of
+ * the pixel's candidate sites composited front to back (sad_dot_alpha, sad_heat_color) */
+__global__ __launch_bounds__(256) void k_tau_heatmap(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                                     float* __restrict__ out, const SadSite* __restrict__ sites,
+                                                     uint32_t n, float tmin, float tmean, float tmax, float dot,
+                                                     int width, int height, int cw, int ch)
+{
+    const int x = (int)(blockIdx.x * 16u + (threadIdx.x & 15u));
+    const int y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    if (x >= width || y >= height) return;
+    const int cell = sad_cand_index(x, y, width, height, cw, ch);
+    const uint4 a = c0[cell], b = c1[cell];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float acc[3] = {0.0f, 0.0f, 0.0f}, acc_a = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t id = ids[k];
+        if (id >= n) continue;
+        const SadSite s = sites[id];
+        if (s.pos_x < 0.0f) continue;
+        const float al = sad_dot_alpha((float)x - s.pos_x, (float)y - s.pos_y, dot);
+        if (al <= 0.0f) continue;
+        float col[3];
+        sad_heat_color(sad_expf(s.log_tau), tmin, tmean, tmax, col);
+        const float blend = al * (1.0f - acc_a);
+        acc[0] += col[0] * blend;
+        acc[1] += col[1] * blend;
+        acc[2] += col[2] * blend;
+        acc_a += blend;
+    }
+    float* o = out + 3u * (uint32_t)(y * width + x);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = acc[c] + 0.05f * (1.0f - acc_a);
+}
+
+int launch_tau_heatmap(const uint32_t* cand0, const uint32_t* cand1, float* out, const SadSite* sites, uint32_t n,
+                       float tmin, float tmean, float tmax, float dot, int width, int height, int cw, int ch,
+                       hipStream_t stream)
+{
+    const dim3 grid(div_up((uint32_t)width, 16u), div_up((uint32_t)height, 16u));
+    hipLaunchKernelGGL(k_tau_heatmap, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(cand0),
+                       reinterpret_cast<const uint4*>(cand1), out, sites, n, tmin, tmean, tmax, dot, width, height,
+                       cw, ch);
+    return launch_status("render_tau_heatmap");
+}
+
 int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                      float* const g[10], const SadSite* sites, float iss, uint32_t n, float* removal,
                      bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
@@ -805,6 +867,26 @@ int sadh_render_voronoi(const uint32_t* cand0, const uint32_t* cand1, float* out
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_voronoi: bad sizes");
     return launch_render(cand0, cand1, output, sites, inv_scale_sq, site_count, width, height, cand_width,
                          cand_height, to_hip(stream));
+}
+
+int sadh_render_cells(const uint32_t* cand0, const uint32_t* cand1, float* output, const SadSite* sites,
+                      float inv_scale_sq, uint32_t site_count, int width, int height, int cand_width,
+                      int cand_height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(cand0 && cand1 && output && sites, "render_cells: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_cells: bad sizes");
+    return launch_render_cells(cand0, cand1, output, sites, inv_scale_sq, site_count, width, height, cand_width,
+                               cand_height, to_hip(stream));
+}
+
+int sadh_render_tau_heatmap(const uint32_t* cand0, const uint32_t* cand1, float* output, const SadSite* sites,
+                            uint32_t site_count, float min_tau, float mean_tau, float max_tau, float dot_radius,
+                            int width, int height, int cand_width, int cand_height, sadh_stream_t stream)
+{
+    SADH_REQUIRE(cand0 && cand1 && output && sites, "render_tau_heatmap: null pointer");
+    SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_tau_heatmap: bad sizes");
+    return launch_tau_heatmap(cand0, cand1, output, sites, site_count, min_tau, mean_tau, max_tau, dot_radius, width,
+                              height, cand_width, cand_height, to_hip(stream));
 }
 
 int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const float* grad_output,

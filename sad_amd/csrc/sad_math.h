@@ -694,4 +694,61 @@ SAD_HD SadSite sad_init_grad_site(const float* target, float px, float py, int p
     return s;
 }
 
+/* ------------------------------------------------------ debug images --
+ * The Metal CLI's cell and tau pictures (--render-cells, --include-debug-mask).
+ *
+ * hashColor, shaders/sad_common.metal:542-565: a colour per site id from three
+ * integer hashes of it (the id, id * 2654435761, id * 1103515245, each mixed
+ * by xor-shifts and a multiply by 0x27d4eb2d), the low 24 bits of each divided
+ * by 2^24 - 1.  The cell picture is the render with these colours in place of
+ * the sites' own. */
+SAD_HD void sad_hash_color(uint32_t id, float rgb[3])
+{
+    uint32_t h = (id ^ 61u) ^ (id >> 16);
+    h += h << 3;
+    h ^= h >> 4;
+    const uint32_t r = h * 0x27d4eb2du;
+    h = id * 2654435761u;
+    h = (h ^ 61u) ^ (h >> 16);
+    h += h << 3;
+    const uint32_t g = h * 0x27d4eb2du;
+    h = id * 1103515245u;
+    h = (h ^ 61u) ^ (h >> 16);
+    h ^= h >> 4;
+    const uint32_t b = h * 0x27d4eb2du;
+    rgb[0] = (float)(r & 0xFFFFFFu) / 16777215.0f;
+    rgb[1] = (float)(g & 0xFFFFFFu) / 16777215.0f;
+    rgb[2] = (float)(b & 0xFFFFFFu) / 16777215.0f;
+}
+
+/* renderCentroidsTauHeatmap, shaders/encoders/render_encoder.metal:136-190:
+ * each candidate site is a dot of radius `dot` (alpha 1 - smoothstep(dot - 1,
+ * dot + 1, distance)), coloured blue -> white -> red as tau runs min -> mean ->
+ * max with the ramp position raised to 0.2, composited front to back in
+ * candidate order over a 0.05 grey. */
+SAD_HD float sad_dot_alpha(float dx, float dy, float dot)
+{
+    const float dist = sqrtf(dx * dx + dy * dy);
+    const float t = sad_clampf((dist - (dot - 1.0f)) / ((dot + 1.0f) - (dot - 1.0f)), 0.0f, 1.0f);
+    return 1.0f - t * t * (3.0f - 2.0f * t);
+}
+
+SAD_HD void sad_heat_color(float tau, float tmin, float tmean, float tmax, float rgb[3])
+{
+    float t;
+    if (tau <= tmean) t = (tmean > tmin) ? (tau - tmin) / (tmean - tmin) : 0.5f;
+    else t = (tmax > tmean) ? (tau - tmean) / (tmax - tmean) : 0.5f;
+    t = sad_clampf(t, 0.0f, 1.0f);
+    const float e = t > 0.0f ? sad_powf_pos(t, 0.2f) : 0.0f;
+    if (tau <= tmean) {            /* mix((0, 0, 1), (1, 1, 1), e) */
+        rgb[0] = e;
+        rgb[1] = e;
+        rgb[2] = 1.0f;
+    } else {                       /* mix((1, 1, 1), (1, 0, 0), e) */
+        rgb[0] = 1.0f;
+        rgb[1] = 1.0f - e;
+        rgb[2] = 1.0f - e;
+    }
+}
+
 #endif /* SAD_MATH_H */

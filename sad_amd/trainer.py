@@ -18,7 +18,7 @@ def fit_plan(opts: dict, width: int, height: int) -> dict:
 
 
 def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, mask: np.ndarray | None = None,
-        want_render: bool = False, **over) -> dict:
+        want_render: bool = False, debug_images: bool = False, **over) -> dict:
     """Run the fit on the current HIP device.
 
     target float32 [H][W][3]; sites float32 [N][10] initial sites (None with
@@ -53,6 +53,11 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
     prender = np.zeros((h, w, 3), np.float32) if (want_render and o.get("packed_psnr")) else None
     if prender is not None:
         res.packed_render = prender.ctypes.data_as(C.POINTER(C.c_float))
+    cells = np.zeros((h, w, 3), np.float32) if debug_images else None
+    heat = np.zeros((h, w, 3), np.float32) if debug_images else None
+    if debug_images:
+        res.cells_render = cells.ctypes.data_as(C.POINTER(C.c_float))
+        res.tau_heatmap = heat.ctypes.data_as(C.POINTER(C.c_float))
     s = to_struct(o)
     _lib.call("sadh_fit", C.byref(s), _lib.ptr(tgt), _lib.ptr(msk), w, h, _lib.ptr(buf), cap, _lib.ptr(logs),
               len(logs), C.byref(res))
@@ -60,6 +65,8 @@ def fit(target: np.ndarray, sites: np.ndarray | None, opts: dict | None = None, 
     out["grad_ms_trace"] = trace
     out["render"] = render
     out["packed_render"] = prender
+    out["cells"] = cells
+    out["tau_heatmap"] = heat
     out["sites"] = buf
     out["log_psnr"] = logs[: min(res.n_logs, len(logs))].copy()
     out["log_ssim"] = slogs[: min(res.n_logs, len(logs))].copy()
@@ -131,16 +138,17 @@ def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list 
 
 def render_sites(sites: np.ndarray, width: int, height: int, cand_passes: int = 16, use_jfa: bool = True,
                  jfa_rounds: int = 1, cand_downscale: int = 1, radius_scale: float = 64.0, radius_probes: int = 0,
-                 inject_count: int = 4, hilbert_window: int = 32, hilbert_probes: int = 1) -> dict:
+                 inject_count: int = 4, hilbert_window: int = 32, hilbert_probes: int = 1, cells: bool = False) -> dict:
     """Render-only job: JFA + VPT candidate build, then the softmax render
     (renderVoronoiFromSites, backends/cuda/src/sad_cuda.cu:1678-1858)."""
     s = np.ascontiguousarray(sites, np.float32)
     out = np.zeros((height, width, 3), np.float32)
+    cell = np.zeros((height, width, 3), np.float32) if cells else None
     cms, rms = C.c_float(), C.c_float()
     _lib.call("sadh_render_sites", _lib.ptr(s), s.shape[0], width, height, cand_passes, int(use_jfa), jfa_rounds,
               cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window, hilbert_probes,
-              _lib.ptr(out), C.byref(cms), C.byref(rms))
-    return dict(image=out, cand_ms=cms.value, render_ms=rms.value)
+              _lib.ptr(out), C.byref(cms), C.byref(rms), _lib.ptr(cell))
+    return dict(image=out, cand_ms=cms.value, render_ms=rms.value, cells=cell)
 
 
 def psnr(a: np.ndarray, b: np.ndarray) -> float:
@@ -166,14 +174,15 @@ def pack_inference(sites: np.ndarray, width: int, height: int):
 def render_packed_sites(packed: np.ndarray, quant: np.ndarray, width: int, height: int, cand_passes: int = 16,
                         use_jfa: bool = True, jfa_rounds: int = 1, cand_downscale: int = 1, radius_scale: float = 64.0,
                         radius_probes: int = 0, inject_count: int = 4, hilbert_window: int = 32,
-                        hilbert_probes: int = 1) -> dict:
+                        hilbert_probes: int = 1, cells: bool = False) -> dict:
     """render_sites on quantised inference sites (sadh_render_packed_sites;
     renderPackedInference, backends/metal/sources/rendering/packed_renderer.swift:15-137)."""
     p = np.ascontiguousarray(packed, np.uint32)
     q = np.ascontiguousarray(quant, np.float32)
     out = np.zeros((height, width, 3), np.float32)
+    cell = np.zeros((height, width, 3), np.float32) if cells else None
     cms, rms = C.c_float(), C.c_float()
     _lib.call("sadh_render_packed_sites", _lib.ptr(p), _lib.ptr(q), p.shape[0], width, height, cand_passes,
               int(use_jfa), jfa_rounds, cand_downscale, radius_scale, radius_probes, inject_count, hilbert_window,
-              hilbert_probes, _lib.ptr(out), C.byref(cms), C.byref(rms))
-    return dict(image=out, cand_ms=cms.value, render_ms=rms.value)
+              hilbert_probes, _lib.ptr(out), C.byref(cms), C.byref(rms), _lib.ptr(cell))
+    return dict(image=out, cand_ms=cms.value, render_ms=rms.value, cells=cell)
