@@ -122,11 +122,8 @@ int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int heigh
                           SadInferenceQuant* quant, hipStream_t stream);
 int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
                             int height, SadSite* out, hipStream_t stream);
-/* nbr_lists (nullable): 8 uint32 per site of scratch for the neighbour lists the passes share */
-int launch_tau_adam_aos(const uint32_t*, const uint32_t*, float*, float*, float*, float*, float, int, int, int,
-                        SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float, uint32_t,
-                        uint32_t, int, int, hipStream_t, uint32_t* nbr_lists = nullptr);
-/* the same tail on double [N][16] lines; requires nbr_lists */
+/* the fit's tail (4 tau passes, the last fused with Adam + clear) on double [N][16] gradient lines;
+ * nbr_lists: 8 uint32 per site of scratch for the neighbour lists the passes share */
 int launch_tau_adam_lines64(const uint32_t*, const uint32_t*, double*, float*, float*, float*, float, int, int, int,
                             SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
                             uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists);

@@ -286,17 +286,6 @@ __global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restr
     adam_site(A, i, g);
 }
 
-/* first pass reading the log_tau column of AoS [N][16] gradient lines */
-__global__ __launch_bounds__(256) void k_tau_pass_strided(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                          const SadSite* __restrict__ sites,
-                                                          const float* __restrict__ g2, float* __restrict__ gout,
-                                                          float* __restrict__ raw_copy, uint32_t n, float lambda,
-                                                          int width, int height, float inv_ds)
-{
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) tau_site(i, c0, c1, sites, g2, g2, gout, raw_copy, n, lambda, width, height, inv_ds, 16u);
-}
-
 __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
                                                      const SadSite* __restrict__ sites,
                                                      const float* __restrict__ graw, const float* __restrict__ gin,
@@ -305,59 +294,6 @@ __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c
 {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) tau_site(i, c0, c1, sites, graw, gin, gout, raw_copy, n, lambda, width, height, inv_ds);
-}
-
-/* ------------------------------------------- last tau pass fused with Adam --
- * The trainer's per-iteration tail (sad_cuda.cu:2333-2389) is four Jacobi
- * passes of tau_diffuse_kernel followed by Adam with the gradient clear.  The
- * fourth pass and Adam touch the same site, so one kernel does both: the pass
- * result stays in a register and feeds Adam directly.  The pass's reads of
- * other sites are `gin` (not written here) and the activity test
- * pos.x >= 0 of neighbours, which Adam cannot change (active positions are
- * clamped to [0, W-1], inactive sites are not written).  Arithmetic and order
- * are those of k_tau_diffuse and k_adam, so the result is bit-identical. */
-__device__ __forceinline__ void tau_last_adam_site(uint32_t i, const uint4* __restrict__ c0,
-                                                   const uint4* __restrict__ c1, const float* __restrict__ graw,
-                                                   const float* __restrict__ gin, float lambda, int width, int height,
-                                                   float inv_ds, const AdamArgs& A)
-{
-    const uint32_t n = A.n;
-    const float2 pos = reinterpret_cast<const float2*>(A.sites + i)[0];
-    float v;
-    if (lambda <= 0.0f || pos.x < 0.0f) {
-        v = gin[i];
-    } else {
-        const float raw = graw[i];
-        int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
-        int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
-        uint4 a = c0[py * width + px], b = c1[py * width + px];
-        uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        float sum = 0.0f;
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            uint32_t nb = ids[k];
-            bool ok = nb < n && nb != i;
-#pragma unroll
-            for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
-            if (ok && A.sites[nb].pos_x >= 0.0f) {
-                sum += gin[nb];
-                cnt += 1u;
-            }
-        }
-        v = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
-    }
-    float g[10];
-    take_gradients(A, i, v, g);
-    adam_site(A, i, g);
-}
-
-__global__ __launch_bounds__(256) void k_tau_last_adam(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                       const float* __restrict__ graw, const float* __restrict__ gin,
-                                                       float lambda, int width, int height, float inv_ds, AdamArgs A)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < A.n) tau_last_adam_site(i, c0, c1, graw, gin, lambda, width, height, inv_ds, A);
 }
 
 /* Batched fit: tau pass `pass` (0-2: log_tau column of the gradient lines ->
@@ -950,43 +886,6 @@ static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw
                        (const float*)tmp2, tmp, n, lambda);
     hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp, lambda, A);
-}
-
-/* The fit's tail on AoS [N][16] float gradient lines: with neighbour lists as
- * tau_adam_lines, else four passes that each walk the candidate field, the
- * last fused with Adam.  Same values as the SoA chain (the buffers differ, not
- * the arithmetic). */
-int launch_tau_adam_aos(const uint32_t* c0, const uint32_t* c1, float* gacc, float* raw, float* tmp, float* tmp2,
-                        float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
-                        float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
-                        float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                        hipStream_t stream, uint32_t* nbr_lists)
-{
-    if (n == 0) return SADH_OK;
-    const float inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
-    const dim3 grid(div_up(n, 256u));
-    const uint4* u0 = reinterpret_cast<const uint4*>(c0);
-    const uint4* u1 = reinterpret_cast<const uint4*>(c1);
-    float* g[10];
-    for (int j = 0; j < 10; ++j) g[j] = gacc + j;
-    AdamArgs A;
-    fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
-              height);
-    A.gstride = 16u;
-    if (nbr_lists) {
-        tau_adam_lines(u0, u1, gacc, raw, tmp, tmp2, lambda, cand_width, cand_height, inv_ds, sites, nbr_lists,
-                       A, stream);
-        return launch_status("tau_diffuse_adam(aos, lists)");
-    }
-    hipLaunchKernelGGL(k_tau_pass_strided, grid, dim3(256), 0, stream, u0, u1, sites, gacc + 2, tmp, raw, n, lambda,
-                       cand_width, cand_height, inv_ds);
-    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp, tmp2, (float*)nullptr, n,
-                       lambda, cand_width, cand_height, inv_ds);
-    hipLaunchKernelGGL(k_tau_diffuse, grid, dim3(256), 0, stream, u0, u1, sites, raw, tmp2, tmp, (float*)nullptr, n,
-                       lambda, cand_width, cand_height, inv_ds);
-    hipLaunchKernelGGL(k_tau_last_adam, grid, dim3(256), 0, stream, u0, u1, raw, tmp, lambda, cand_width,
-                       cand_height, inv_ds, A);
-    return launch_status("tau_diffuse_adam(aos)");
 }
 
 /* The same tail on double gradient lines (the fit's cross-tile sums), which
