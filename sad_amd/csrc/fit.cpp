@@ -356,8 +356,11 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         /* the SSIM loss term reads a render of the current state (sad_cuda.cu:2307-2318) */
         if (o.ssim_weight > 0.0f) SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
+        /* the tail's neighbour lists are written by extra blocks of the gradient launch when the tail uses the
+         * gradient's field; after a candidate update on the second stream the first tau pass writes them */
         SAD_TRY(launch_gradients_lines64(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width,
-                                         height, cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f));
+                                         height, cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f,
+                                         pending ? nullptr : tau_nbr, lambda, ds));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             gbytes.push_back(48.0 * np + 120.0 * n + (prune ? 8.0 * n : 0.0));
@@ -370,7 +373,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
         SAD_TRY(launch_tau_adam_lines64(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites,
                                         d_adam, lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr));
+                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr, !pending));
 
         if (prune) {
             n_prune++;

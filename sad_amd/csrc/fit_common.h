@@ -126,11 +126,13 @@ int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant*
  * nbr_lists: 8 uint32 per site of scratch for the neighbour lists the passes share */
 int launch_tau_adam_lines64(const uint32_t*, const uint32_t*, double*, float*, float*, float*, float, int, int, int,
                             SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
-                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists);
+                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists, bool lists_built);
 /* the fit's double cross-tile sums: gradient lines and removal deltas, site statistics */
+/* nbr_lists (nullable): the same launch also writes the tau passes' neighbour lists on this field */
 int launch_gradients_lines64(const uint32_t*, const uint32_t*, const float*, const float*, double*, const SadSite*,
                              float, uint32_t, double*, bool, int, int, int, int, hipStream_t,
-                             const float* rendered = nullptr, float ssim_weight = 0.0f);
+                             const float* rendered = nullptr, float ssim_weight = 0.0f, uint32_t* nbr_lists = nullptr,
+                             float nbr_lambda = 0.0f, int ds = 1);
 int launch_site_stats64(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
                         double* const[8], int, int, int, int, hipStream_t);
 int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const double* const[8], uint32_t, const float*,

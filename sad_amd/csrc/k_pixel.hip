@@ -192,6 +192,11 @@ struct PixelArgs {
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3] */
     uint32_t cells;            /* RENDER: 1 = each site's colour is sad_hash_color of its id (cell picture) */
+    /* GRAD: the first list_blocks blocks of the launch write the tau passes' neighbour lists (tau_list_only)
+     * instead of a tile; they read only sites and the candidate field, which the gradient does not change */
+    uint4* nbr;
+    float nbr_lambda, nbr_inv_ds;
+    uint32_t list_blocks;
     float* g[10];              /* DST_SOA: value j of site i accumulates at g[j] + i * gstride */
     float* gline;              /* DST_LINES: value j of site i accumulates at gline + i * gstride + j */
     double* gline64;           /* DST_LINES64: value j of site i at gline64 + i * 16 + j */
@@ -645,7 +650,16 @@ template <int MODE, bool REMOVAL, int H, int DST>
 __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
     __shared__ PixelLds<MODE, REMOVAL, H> L;
-    pixel_tile<MODE, REMOVAL, H, DST>(A, xcd_tile(blockIdx.x, A.ntiles), L);
+    if constexpr (MODE == MODE_GRAD) {
+        if (blockIdx.x < A.list_blocks) {
+            const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+            if (i < A.n)
+                tau_list_only(i, reinterpret_cast<const uint4*>(A.cand0), reinterpret_cast<const uint4*>(A.cand1),
+                              A.sites, A.n, A.nbr_lambda, A.cw, A.ch, A.nbr_inv_ds, A.nbr);
+            return;
+        }
+    }
+    pixel_tile<MODE, REMOVAL, H, DST>(A, xcd_tile(blockIdx.x - A.list_blocks, A.ntiles), L);
 }
 
 /* Batched fit: one launch over the concatenated tiles of every job. */
@@ -693,7 +707,8 @@ static int launch_pixel(PixelArgs& A, hipStream_t stream, const char* what)
     A.tiles_x = div_up((uint32_t)A.width, 16u);
     A.ntiles = A.tiles_x * div_up((uint32_t)A.height, 16u);
     A.inv_scale = sqrtf(A.iss);
-    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS, DST>), dim3(A.ntiles), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL((k_pixel<MODE, REMOVAL, HASH_SLOTS, DST>), dim3(A.ntiles + A.list_blocks), dim3(256), 0,
+                       stream, A);
     return launch_status(what);
 }
 
@@ -799,7 +814,8 @@ int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const flo
 int launch_gradients_lines64(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                              double* gacc, const SadSite* sites, float iss, uint32_t n, double* removal,
                              bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
-                             const float* rendered, float ssim_weight)
+                             const float* rendered, float ssim_weight, uint32_t* nbr_lists, float nbr_lambda,
+                             int ds)
 {
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
@@ -809,6 +825,13 @@ int launch_gradients_lines64(const uint32_t* cand0, const uint32_t* cand1, const
     A.gstride = 16u;
     A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
+    if (nbr_lists && n > 0u) {
+        /* a multiple of 8 blocks keeps the tiles' XCD grouping aligned */
+        A.list_blocks = div_up(div_up(n, 256u), 8u) * 8u;
+        A.nbr = reinterpret_cast<uint4*>(nbr_lists);
+        A.nbr_lambda = nbr_lambda;
+        A.nbr_inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
+    }
     if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_LINES64>(A, stream, "compute_gradients(lines64)");
     return launch_pixel<MODE_GRAD, false, DST_LINES64>(A, stream, "compute_gradients(lines64)");
 }

@@ -192,7 +192,6 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
  * later passes read that list instead of chasing position -> home cell ->
  * candidates -> neighbour activity again.  Sums run in the same order as in
  * tau_site, so the results are bit-identical. */
-#define NBR_COPY 0xFFFFFFFEu
 
 template <typename TR, typename TI>
 __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
@@ -253,6 +252,19 @@ __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restri
         }
     }
     return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
+}
+
+/* first pass from lists built beforehand (during the gradient launch), reading
+ * the log_tau column of the gradient lines; saves raw */
+template <typename T>
+__global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __restrict__ nbr,
+                                                              const T* __restrict__ g2, float* __restrict__ gout,
+                                                              float* __restrict__ raw_copy, uint32_t n, float lambda)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    raw_copy[i] = (float)g2[i * 16u];
+    gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
 }
 
 template <typename T>
@@ -866,19 +878,23 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
 }
 
 /* The list-driven tail on [N][16] gradient lines of type T: pass 1 reads the
- * strided log_tau column (raw saved) and writes the neighbour lists; passes
- * 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam, which reads and clears
- * each site's line. */
+ * strided log_tau column (raw saved) and, unless lists_built, writes the
+ * neighbour lists; passes 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam,
+ * which reads and clears each site's line. */
 template <typename T>
 static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw, float* tmp, float* tmp2,
                            float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
-                           uint32_t* nbr_lists, const AdamArgs& A, hipStream_t stream)
+                           uint32_t* nbr_lists, bool lists_built, const AdamArgs& A, hipStream_t stream)
 {
     const uint32_t n = A.n;
     const dim3 grid(div_up(n, 256u));
     uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
     const T* g2 = gacc + 2;
-    hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
+    if (lists_built)
+        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(256), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
+                           lambda);
+    else
+        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
                            raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
     hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp, tmp2, n, lambda);
@@ -894,7 +910,7 @@ int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc
                             float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                             float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                             float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
-                            hipStream_t stream, uint32_t* nbr_lists)
+                            hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
 {
     if (n == 0) return SADH_OK;
     if (!nbr_lists) return fail(SADH_ERR_INVALID, "tau_diffuse_adam(lines64): neighbour lists are required");
@@ -904,7 +920,7 @@ int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc
               height);
     A.g64 = gacc;
     tau_adam_lines(reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), gacc, raw, tmp, tmp2,
-                   lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, A,
+                   lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, lists_built, A,
                    stream);
     return launch_status("tau_diffuse_adam(lines64)");
 }

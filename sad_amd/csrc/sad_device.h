@@ -66,6 +66,40 @@ __device__ __forceinline__ float group8_max(float v)
     return v;
 }
 
+/* Tau-pass neighbour list of site i (tau_diffuse_kernel's neighbour set,
+ * tau_diffusion.cu:27-89): the unique active ids other than i in the candidate
+ * list of the site's home cell, as 8 ids in candidate order (SAD_EMPTY for ids
+ * that do not count), or NBR_COPY in slot 0 when the pass copies its input
+ * through (lambda <= 0 or i inactive). */
+#define NBR_COPY 0xFFFFFFFEu
+
+__device__ __forceinline__ void tau_list_only(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
+                                              const SadSite* __restrict__ sites, uint32_t n, float lambda, int width,
+                                              int height, float inv_ds, uint4* __restrict__ nbr)
+{
+    const float2 pos = reinterpret_cast<const float2*>(sites + i)[0];
+    if (lambda <= 0.0f || pos.x < 0.0f) {
+        nbr[2u * i] = make_uint4(NBR_COPY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        nbr[2u * i + 1u] = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        return;
+    }
+    const int px = sad_clampi((int)(pos.x * inv_ds + 0.5f), 0, width - 1);
+    const int py = sad_clampi((int)(pos.y * inv_ds + 0.5f), 0, height - 1);
+    const uint4 a = c0[py * width + px], b = c1[py * width + px];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t keep[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t nb = ids[k];
+        bool ok = nb < n && nb != i;
+#pragma unroll
+        for (int j = 0; j < k; ++j) ok = ok && ids[j] != nb;
+        keep[k] = (ok && sites[nb].pos_x >= 0.0f) ? nb : SAD_EMPTY;
+    }
+    nbr[2u * i] = make_uint4(keep[0], keep[1], keep[2], keep[3]);
+    nbr[2u * i + 1u] = make_uint4(keep[4], keep[5], keep[6], keep[7]);
+}
+
 /* Block index -> tile index, grouping consecutive tiles on one XCD.  Workgroups
  * are dealt to the 8 XCDs round-robin, so without the remap neighbouring tiles
  * (which share most candidate sites) land in different L2s. */
