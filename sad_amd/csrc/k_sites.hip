@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __res
                                                               const T* __restrict__ g2, float* __restrict__ gout,
                                                               float* __restrict__ raw_copy, uint32_t n, float lambda)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     raw_copy[i] = (float)g2[i * 16u];
     gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict_
                                                         uint32_t n, float lambda, int width, int height, float inv_ds,
                                                         uint4* __restrict__ nbr, uint32_t in_stride)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) tau_site_list(i, c0, c1, sites, g2, g2, gout, raw_copy, n, lambda, width, height, inv_ds, in_stride,
                              nbr);
 }
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_tau_list_pass(const uint4* __restrict__
                                                        const float* __restrict__ gin, float* __restrict__ gout,
                                                        uint32_t n, float lambda)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) gout[i] = tau_from_list(i, nbr, graw, gin, lambda);
 }
 
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restr
                                                             const float* __restrict__ graw,
                                                             const float* __restrict__ gin, float lambda, AdamArgs A)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     float g[10];
     take_gradients(A, i, tau_from_list(i, nbr, graw, gin, lambda), g);
@@ -886,21 +886,24 @@ static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw
                            float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
                            uint32_t* nbr_lists, bool lists_built, const AdamArgs& A, hipStream_t stream)
 {
+    /* 64-thread blocks: these passes are dependent gathers over a few 10k sites, and smaller blocks spread the
+     * same waves over more CUs and their caches */
+    constexpr uint32_t TB = 64u;
     const uint32_t n = A.n;
-    const dim3 grid(div_up(n, 256u));
+    const dim3 grid(div_up(n, TB));
     uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
     const T* g2 = gacc + 2;
     if (lists_built)
-        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(256), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
+        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(TB), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
                            lambda);
     else
-        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(256), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
+        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(TB), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
                            raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp, tmp2, n, lambda);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp2, tmp, n, lambda);
-    hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(256), 0, stream, (const uint4*)nbr, (const float*)raw,
+    hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
                        (const float*)tmp, lambda, A);
 }
 
