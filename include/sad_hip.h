@@ -321,6 +321,10 @@ typedef struct SadhFitOptions {
     int ssim_metric;                  /* 1: log the mean SSIM beside the PSNR (--ssim, sad_cuda.cu:2432-2447) */
     int packed_psnr;                  /* 1: after the fit, render the quantised inference sites with the final
                                          candidates and report that PSNR (--packed-psnr, training.swift:676-703) */
+    int graph_ticks;                  /* sadh_fit_batch and the server: 1 replays each steady iteration (no
+                                         candidate update, densify, prune, log or profiling) as a captured
+                                         hipGraph of its gradient and tau/Adam launches; the same results.
+                                         No reference counterpart; sadh_fit ignores it */
 } SadhFitOptions;
 
 typedef struct SadhFitResult {
@@ -369,6 +373,7 @@ typedef struct SadhBatchResult {
     int grad_launches;
     double grad_bytes;                /* summed algorithmic bytes of those launches */
     double total_pixels;              /* sum of W*H over the jobs */
+    int graph_ticks, graph_captures;  /* graph_ticks: iterations replayed as a graph, graphs captured */
 } SadhBatchResult;
 
 int sadh_fit_plan(const SadhFitOptions* opts, int width, int height, float* densify_percentile,

@@ -275,9 +275,17 @@ struct Engine {
     std::vector<hipEvent_t> gev;
     std::vector<double> gbytes;
     Clock::time_point t_origin = Clock::now();
+    /* graph_ticks: one executable graph per ring slot, valid while its epoch is current.  The epoch
+     * moves whenever a launch argument of a steady iteration would change: a new job table, a site
+     * count changed by densify, or (with the SSIM render) a candidate field flipped on the host. */
+    hipGraphExec_t tick_exec[RING] = {};
+    uint64_t tick_epoch[RING] = {};
+    uint64_t epoch = 1;
+    int graph_replays = 0, graph_captures = 0;
 
     ~Engine()
     {
+        for (auto& g : tick_exec) if (g) (void)hipGraphExecDestroy(g);
         for (auto e : gev) (void)hipEventDestroy(e);
         for (auto& e : slot_done) if (e) (void)hipEventDestroy(e);
         if (steps_ready) (void)hipEventDestroy(steps_ready);
@@ -345,6 +353,61 @@ struct Engine {
         }
         if (nj > 0) HIP_TRY(hipMemcpy(d_jobs, desc.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice));
         table_dirty = false;
+        epoch++;
+        return SADH_OK;
+    }
+
+    /* the launches of a steady iteration: the SSIM renders (when the loss has that term), the batched
+     * gradient and the batched tau diffusion + Adam; every per-iteration value comes from the device
+     * step table `ds` and the job table, so the same launches replay as a graph */
+    int steady_launches(int nj, const JobStep* ds, hipStream_t S)
+    {
+        if (o.ssim_weight > 0.0f)
+            for (int j = 0; j < nj; ++j) {
+                FitRun& J = *fits[j];
+                SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
+                                      J.ch, S));
+            }
+        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, false, S));
+        BatchStep st = adam_step();
+        return launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, st, S);
+    }
+
+    BatchStep adam_step() const
+    {
+        BatchStep st = {};
+        st.lr_pos = o.lr_pos * o.lr_scale; st.lr_tau = o.lr_tau * o.lr_scale; st.lr_radius = o.lr_radius * o.lr_scale;
+        st.lr_color = o.lr_color * o.lr_scale; st.lr_dir = o.lr_dir * o.lr_scale; st.lr_aniso = o.lr_aniso * o.lr_scale;
+        st.beta1 = o.beta1; st.beta2 = o.beta2; st.eps = o.eps;
+        return st;
+    }
+
+    /* replay slot `slot`'s graph of the steady launches, capturing it first when its epoch is old */
+    int graph_tick(int slot, int nj, const JobStep* ds)
+    {
+        hipStream_t S = sp.main;
+        if (!tick_exec[slot] || tick_epoch[slot] != epoch) {
+            if (tick_exec[slot]) {
+                HIP_TRY(hipGraphExecDestroy(tick_exec[slot]));
+                tick_exec[slot] = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed));
+            const int rc = steady_launches(nj, ds, S);
+            const hipError_t ec = hipStreamEndCapture(S, &g);
+            if (rc != SADH_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(ec);
+            const hipError_t ei = hipGraphInstantiate(&tick_exec[slot], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_TRY(ei);
+            tick_epoch[slot] = epoch;
+            graph_captures++;
+        }
+        HIP_TRY(hipGraphLaunch(tick_exec[slot], S));
+        graph_replays++;
         return SADH_OK;
     }
 
@@ -362,7 +425,6 @@ struct Engine {
         if (table_dirty) SAD_TRY(rebuild_table());
         hipStream_t S = sp.main;
         const int nj = (int)fits.size();
-        const int passes = std::max(1, o.cand_update_passes);
         int eff_ps = o.prune_start;
         if (o.densify_enabled && !o.prune_during_densify && o.prune_start < o.densify_end) eff_ps = o.densify_end;
         /* per-fit flags and step table (a ring slot whose previous tick has finished) */
@@ -370,7 +432,7 @@ struct Engine {
         HIP_TRY(hipEventSynchronize(slot_done[slot]));
         JobStep* hs = h_steps + (size_t)slot * steps_cap;
         JobStep* ds = d_steps + (size_t)slot * steps_cap;
-        bool any_upd = false, any_dens_upd = false, any_prune = false, any_log = false;
+        bool any_upd = false, any_dens_upd = false, any_prune = false, any_log = false, any_dens = false;
         for (int j = 0; j < nj; ++j) {
             FitRun& J = *fits[j];
             const int it = J.iter;
@@ -384,6 +446,7 @@ struct Engine {
             J.log = it % std::max(1, o.log_freq) == 0 || it == J.iterations - 1;
             any_upd = any_upd || J.upd;
             any_dens_upd = any_dens_upd || (J.upd && J.dens);
+            any_dens = any_dens || (J.dens && J.pairs_count > 0);
             any_prune = any_prune || J.prune;
             any_log = any_log || J.log;
             const float blend = (float)it / (float)std::max(1, J.iterations);
@@ -395,8 +458,23 @@ struct Engine {
             hs[j].vpt = J.upd ? 1u : 0u;
         }
         HIP_TRY(hipMemcpyAsync(ds, hs, sizeof(JobStep) * nj, hipMemcpyHostToDevice, S));
-        HIP_TRY(hipEventRecord(steps_ready, S));
+        if (any_upd) HIP_TRY(hipEventRecord(steps_ready, S));
 
+        const bool steady = o.graph_ticks && !any_upd && !any_dens && !any_prune && !any_log && !o.profile_gradients;
+        if (steady)
+            SAD_TRY(graph_tick(slot, nj, ds));
+        else
+            SAD_TRY(eager_tick(nj, ds, any_upd, any_dens_upd, any_prune));
+        HIP_TRY(hipEventRecord(slot_done[slot], S));
+        tick_no++;
+        return finish_tick(any_log, done);
+    }
+
+    /* a tick's launches when some fit updates candidates, densifies, prunes or logs (or graphs are off) */
+    int eager_tick(int nj, const JobStep* ds, bool any_upd, bool any_dens_upd, bool any_prune)
+    {
+        hipStream_t S = sp.main;
+        const int passes = std::max(1, o.cand_update_passes);
         std::vector<uint32_t> late;
         bool pending = false;
         if (any_upd) {
@@ -439,6 +517,7 @@ struct Engine {
             }
             SAD_TRY(flip(now_flip, S));
             for (uint32_t j : now_flip) fits[j]->cur ^= 1u;
+            if (o.ssim_weight > 0.0f && (!now_flip.empty() || !late.empty())) epoch++;
             if (!any_dens_upd) {
                 HIP_TRY(hipEventRecord(sp.cand_done, sp.cand));
                 pending = true;
@@ -471,9 +550,11 @@ struct Engine {
                 upd_flip.push_back(0u);
             }
         }
-        if (!upd_idx.empty())
+        if (!upd_idx.empty()) {
             SAD_TRY(launch_update_jobs(d_jobs, upd_idx.data(), upd_n.data(), upd_flip.data(), (uint32_t)upd_idx.size(),
                                        S));
+            epoch++;
+        }
         if (any_prune)
             for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear64(&fits[j]->removal, 1, fits[j]->n, S));
 
@@ -504,11 +585,7 @@ struct Engine {
         SAD_TRY(flip(late, S));
         for (uint32_t j : late) fits[j]->cur ^= 1u;
 
-        BatchStep st = {};
-        st.lr_pos = o.lr_pos * o.lr_scale; st.lr_tau = o.lr_tau * o.lr_scale; st.lr_radius = o.lr_radius * o.lr_scale;
-        st.lr_color = o.lr_color * o.lr_scale; st.lr_dir = o.lr_dir * o.lr_scale; st.lr_aniso = o.lr_aniso * o.lr_scale;
-        st.beta1 = o.beta1; st.beta2 = o.beta2; st.eps = o.eps;
-        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, st, S));
+        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, adam_step(), S));
 
         for (int j = 0; j < nj; ++j) {
             FitRun& J = *fits[j];
@@ -524,9 +601,14 @@ struct Engine {
                 J.active_est = std::max(0, J.active_est - k);
             }
         }
-        HIP_TRY(hipEventRecord(slot_done[slot], S));
-        tick_no++;
+        return SADH_OK;
+    }
 
+    /* the log renders and metrics, then the iteration count; retires finished fits */
+    int finish_tick(bool any_log, std::vector<std::unique_ptr<FitRun>>& done)
+    {
+        hipStream_t S = sp.main;
+        const int nj = (int)fits.size();
         if (any_log) {
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
@@ -675,6 +757,8 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
         bres->train_seconds = secs;
         SAD_TRY(E.grad_totals(&bres->grad_kernel_ms, &bres->grad_launches, &bres->grad_bytes));
         bres->total_pixels = px;
+        bres->graph_ticks = E.graph_replays;
+        bres->graph_captures = E.graph_captures;
     }
     return SADH_OK;
 }

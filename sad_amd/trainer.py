@@ -79,7 +79,8 @@ def fit_batch(targets: list, sites: list, opts: dict | None = None, masks: list 
     """Fit many images in lock step with one launch per kernel per iteration
     (sadh_fit_batch).  targets[i] float32 [H][W][3], sites[i] float32 [N_i][10];
     all jobs share the options.  Returns {"jobs": [per-job dicts as fit()],
-    "train_seconds", "grad_kernel_ms", "grad_launches", "grad_bytes", "total_pixels"}."""
+    "train_seconds", "grad_kernel_ms", "grad_launches", "grad_bytes", "total_pixels", "graph_ticks",
+    "graph_captures"}."""
     o = dict(opts or fit_options())
     o.update(over)
     if o["init_mode"] != 0:

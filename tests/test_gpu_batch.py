@@ -80,3 +80,26 @@ def test_fits_repeat_exactly():
         for r in (runs[1][i], single[0], single[1]):
             assert np.array_equal(r["sites"].view(np.uint32), runs[0][i]["sites"].view(np.uint32))
             assert np.array_equal(np.asarray(r["log_psnr"]), np.asarray(runs[0][i]["log_psnr"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ssim_weight", [0.0, 0.5])
+def test_graph_ticks_equal_eager_ticks(ssim_weight):
+    """graph_ticks replays each steady iteration (no candidate update, densify,
+    prune or log) as a captured hipGraph.  Those replays run between eager
+    iterations, host synchronisations and recaptures (a densify changes a site
+    count; with the SSIM render a candidate flip changes a launch argument), and
+    every job ends bit-identical to the same batch without graphs."""
+    specs = [(150, 100, 62, 500), (90, 170, 63, 400), (64, 48, 64, 120)]
+    tg, st = _jobs(specs)
+    opts = config.fit_options(iterations=130, target_bpp=1.5, densify_freq=20, prune_freq=20, densify_start=20,
+                              prune_start=40, max_sites=2000, log_freq=30, ssim_weight=ssim_weight)
+    eager = trainer.fit_batch(tg, st, opts)
+    graph = trainer.fit_batch(tg, st, dict(opts, graph_ticks=1))
+    print("graph ticks", graph["graph_ticks"], "captures", graph["graph_captures"])
+    assert eager["graph_ticks"] == 0 and graph["graph_ticks"] >= 80
+    assert 4 <= graph["graph_captures"] < graph["graph_ticks"]
+    for e, g in zip(eager["jobs"], graph["jobs"]):
+        assert np.array_equal(g["sites"].view(np.uint32), e["sites"].view(np.uint32))
+        assert np.array_equal(g["log_psnr"], e["log_psnr"])
+        assert (g["n_densify"], g["n_prune"], g["actual_sites"]) == (e["n_densify"], e["n_prune"], e["actual_sites"])
