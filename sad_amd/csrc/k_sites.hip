@@ -254,19 +254,6 @@ __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restri
     return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
-/* first pass from lists built beforehand (during the gradient launch), reading
- * the log_tau column of the gradient lines; saves raw */
-template <typename T>
-__global__ __launch_bounds__(256) void k_tau_list_first_built(const uint4* __restrict__ nbr,
-                                                              const T* __restrict__ g2, float* __restrict__ gout,
-                                                              float* __restrict__ raw_copy, uint32_t n, float lambda)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    raw_copy[i] = (float)g2[i * 16u];
-    gout[i] = tau_from_list(i, nbr, g2, g2, lambda, 16u);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
                                                         const SadSite* __restrict__ sites, const T* __restrict__ g2,
@@ -295,6 +282,75 @@ __global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restr
     if (i >= A.n) return;
     float g[10];
     take_gradients(A, i, tau_from_list(i, nbr, graw, gin, lambda), g);
+    adam_site(A, i, g);
+}
+
+/* Two passes in one launch.  Site i's pass-(k+1) value needs its neighbours'
+ * pass-k values; each is recomputed here from the neighbour's own list with the
+ * same arithmetic as its separate pass (tau_from_list), so the result has the
+ * same bits as two launches.  Eight lanes serve one site, lane k recomputing
+ * neighbour k's value; the group's first lane sums them in slot order. */
+template <typename TR, typename TI>
+__device__ __forceinline__ float tau_two_passes8(uint32_t i, uint32_t lane, const uint4* __restrict__ nbr,
+                                                 const TR* __restrict__ graw, const TI* __restrict__ gin,
+                                                 float lambda, uint32_t stride)
+{
+    const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
+    const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t mine = ids[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) mine = lane == (uint32_t)k ? ids[k] : mine;
+    const bool copy = a.x == NBR_COPY;
+    float v = 0.0f;
+    if (!copy && mine != SAD_EMPTY) v = tau_from_list(mine, nbr, graw, gin, lambda, stride);
+    const int base = (int)(threadIdx.x & ~7u);
+    float vals[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) vals[k] = __shfl(v, base + k, 64);
+    if (copy) return (float)gin[i * stride];
+    float sum = 0.0f;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (ids[k] != SAD_EMPTY) {
+            sum += vals[k];
+            cnt += 1u;
+        }
+    }
+    return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
+}
+
+/* passes 1 and 2 from lists built during the gradient launch: reads the
+ * log_tau column of the gradient lines, saves raw, writes the pass-2 value.
+ * 8 threads per site; blockDim is a multiple of 64. */
+template <typename T>
+__global__ __launch_bounds__(256) void k_tau_list_12_built(const uint4* __restrict__ nbr, const T* __restrict__ g2,
+                                                           float* __restrict__ gout, float* __restrict__ raw_copy,
+                                                           uint32_t n, float lambda)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t >> 3, lane = t & 7u;
+    if (i >= n) return;                        /* whole groups of 8 leave together */
+    const float v = tau_two_passes8(i, lane, nbr, g2, g2, lambda, 16u);
+    if (lane == 0u) {
+        raw_copy[i] = (float)g2[i * 16u];
+        gout[i] = v;
+    }
+}
+
+/* passes 3 and 4 from the pass-2 values, then Adam (which reads and clears the
+ * site's gradient line); 8 threads per site */
+__global__ __launch_bounds__(256) void k_tau_list_34_adam(const uint4* __restrict__ nbr,
+                                                          const float* __restrict__ graw,
+                                                          const float* __restrict__ gin, float lambda, AdamArgs A)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t >> 3, lane = t & 7u;
+    if (i >= A.n) return;
+    const float tau = tau_two_passes8(i, lane, nbr, graw, gin, lambda, 1u);
+    if (lane != 0u) return;
+    float g[10];
+    take_gradients(A, i, tau, g);
     adam_site(A, i, g);
 }
 
@@ -891,20 +947,22 @@ static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw
     constexpr uint32_t TB = 64u;
     const uint32_t n = A.n;
     const dim3 grid(div_up(n, TB));
+    const dim3 grid8(div_up(n, TB / 8u));          /* 8 threads per site */
     uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
     const T* g2 = gacc + 2;
-    if (lists_built)
-        hipLaunchKernelGGL(k_tau_list_first_built<T>, grid, dim3(TB), 0, stream, (const uint4*)nbr, g2, tmp, raw, n,
+    if (lists_built) {
+        /* passes 1+2, then 3+4 with Adam: two launches */
+        hipLaunchKernelGGL(k_tau_list_12_built<T>, grid8, dim3(TB), 0, stream, (const uint4*)nbr, g2, tmp2, raw, n,
                            lambda);
-    else
+    } else {
+        /* pass 1 writes the lists, so pass 2 cannot share its launch */
         hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(TB), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
                            raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
-                       (const float*)tmp, tmp2, n, lambda);
-    hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
-                       (const float*)tmp2, tmp, n, lambda);
-    hipLaunchKernelGGL(k_tau_list_last_adam, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
-                       (const float*)tmp, lambda, A);
+        hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
+                           (const float*)tmp, tmp2, n, lambda);
+    }
+    hipLaunchKernelGGL(k_tau_list_34_adam, grid8, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
+                       (const float*)tmp2, lambda, A);
 }
 
 /* The same tail on double gradient lines (the fit's cross-tile sums), which
