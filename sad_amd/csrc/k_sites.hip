@@ -934,9 +934,10 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
 }
 
 /* The list-driven tail on [N][16] gradient lines of type T: pass 1 reads the
- * strided log_tau column (raw saved) and, unless lists_built, writes the
- * neighbour lists; passes 2-3 ping-pong tmp -> tmp2 -> tmp; pass 4 feeds Adam,
- * which reads and clears each site's line. */
+ * strided log_tau column (raw saved).  With lists_built (written during the
+ * gradient launch) passes 1+2 are one launch into tmp2; otherwise pass 1 writes
+ * the neighbour lists into tmp and pass 2 reads them into tmp2.  Passes 3+4 are
+ * one launch that feeds Adam, which reads and clears each site's line. */
 template <typename T>
 static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw, float* tmp, float* tmp2,
                            float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
