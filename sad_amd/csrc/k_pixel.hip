@@ -271,18 +271,30 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     }
     __syncthreads();
 
-    /* stage 1: this pixel's candidate ids into the tile hash */
+    /* stage 1: this pixel's candidate ids into the tile hash.  The mask, the
+     * candidates and the pixel's target (or upstream gradient, read in stage 3)
+     * are loaded together, so their latencies overlap; a masked-out pixel then
+     * drops its candidates. */
     bool act = inb;
-    if (act && (MODE == MODE_GRAD || MODE == MODE_STATS)) act = A.mask[pix] > 0.0f;
     uint32_t ids[8];
     uint16_t slots[8];
+    float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
     {
-        uint4 a = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY), b = a;
-        if (act) {
+        const uint4 none = make_uint4(SAD_EMPTY, SAD_EMPTY, SAD_EMPTY, SAD_EMPTY);
+        uint4 a = none, b = none;
+        float mk = 1.0f;
+        if (inb) {
             int cell = sad_cand_index(x, y, A.width, A.height, A.cw, A.ch);
             a = reinterpret_cast<const uint4*>(A.cand0)[cell];
             b = reinterpret_cast<const uint4*>(A.cand1)[cell];
+            if constexpr (MODE == MODE_GRAD || MODE == MODE_STATS) mk = A.mask[pix];
+            if constexpr (MODE != MODE_RENDER) {
+                const float* qp = (MODE == MODE_BWD ? A.grad_out : A.target) + 3u * pix;
+                q0 = qp[0]; q1 = qp[1]; q2 = qp[2];
+            }
         }
+        if (MODE == MODE_GRAD || MODE == MODE_STATS) act = act && mk > 0.0f;
+        if (!act) { a = none; b = none; }
         ids[0] = a.x; ids[1] = a.y; ids[2] = a.z; ids[3] = a.w;
         ids[4] = b.x; ids[5] = b.y; ids[6] = b.z; ids[7] = b.w;
 #pragma unroll
@@ -408,11 +420,10 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     float tr = 0.0f, tg = 0.0f, tb = 0.0f;
     if (contrib) {
         if (MODE == MODE_BWD) {
-            const float* go = A.grad_out + 3u * pix;
-            gr = go[0]; gg = go[1]; gb = go[2];
+            gr = q0; gg = q1; gb = q2;
         } else {
             const float* tp = A.target + 3u * pix;
-            tr = tp[0]; tg = tp[1]; tb = tp[2];
+            tr = q0; tg = q1; tb = q2;
             float dr = pr - tr, dg = pgr - tg, db = pb - tb;
             gr = 2.0f * dr; gg = 2.0f * dg; gb = 2.0f * db;
             err = dr * dr + dg * dg + db * db;
