@@ -109,6 +109,7 @@ struct TileLds : TileSites<H> {
     uint32_t wcount[2][H];
     uint16_t cstart[H];        /* first chunk of each slot */
     uint32_t n_chunks;
+    uint32_t stot[4][2];       /* stage-4 entry and chunk totals of each scanning wave */
     uint32_t n_ovf;            /* pairs whose id found no hash slot */
     uint32_t hstate;           /* nonzero: an id found no slot in its 8 probes */
     /* per pixel: slot of each candidate rank (SLOT8_NONE if unused); read only to sum a site's weight over
@@ -532,35 +533,57 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         }
     }
 
-    /* stage 4 (wave 0): entry offsets and chunk offsets = exclusive scans of the
-     * per-slot entry and chunk counts; chunk -> slot table */
-    if (tid < 64u) {
-        uint32_t run = 0u, crun = 0u;
-        for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
-            const uint32_t s = s0 + tid;
+    /* stage 4 (waves 0..H/64-1, 64 slots each): entry offsets and chunk offsets =
+     * exclusive scans of the per-slot entry and chunk counts; chunk -> slot table.
+     * Each wave scans its slots, posts its totals, and after a barrier adds the
+     * totals of the waves before it. */
+    static_assert(H % 64 == 0 && H <= 256, "stage 4 scans 64 slots per wave");
+    {
+        const uint32_t lane = tid & 63u;
+        const bool scan = tid < (uint32_t)H;
+        uint32_t c = 0u, nc = 0u, v = 0u, vc = 0u, c0 = 0u, c01 = 0u, c012 = 0u;
+        if (scan) {
             /* per-wave counts (at most 512 each) -> per-wave offsets within the slot, and the total */
-            const uint32_t w01 = L.wcount[0][s], w23 = L.wcount[1][s];
-            const uint32_t c0 = w01 & 0xFFFFu, c01 = c0 + (w01 >> 16), c012 = c01 + (w23 & 0xFFFFu);
-            const uint32_t c = c012 + (w23 >> 16);
-            L.wcount[0][s] = c0 << 16;
-            L.wcount[1][s] = c01 | (c012 << 16);
-            L.count[s] = c;
-            const uint32_t nc = (c + CHUNK - 1u) / CHUNK;
-            uint32_t v = c, vc = nc;
+            const uint32_t w01 = L.wcount[0][tid], w23 = L.wcount[1][tid];
+            c0 = w01 & 0xFFFFu;
+            c01 = c0 + (w01 >> 16);
+            c012 = c01 + (w23 & 0xFFFFu);
+            c = c012 + (w23 >> 16);
+            nc = (c + CHUNK - 1u) / CHUNK;
+            v = c;
+            vc = nc;
 #pragma unroll
             for (uint32_t off = 1; off < 64u; off <<= 1) {
                 uint32_t o = (uint32_t)__shfl_up((int)v, off);
                 uint32_t oc = (uint32_t)__shfl_up((int)vc, off);
-                if (tid >= off) { v += o; vc += oc; }
+                if (lane >= off) { v += o; vc += oc; }
             }
+            if (lane == 63u) {
+                L.stot[wv][0] = v;
+                L.stot[wv][1] = vc;
+            }
+        }
+        __syncthreads();
+        if (scan) {
+            uint32_t run = 0u, crun = 0u;
+            for (uint32_t j = 0; j < wv; ++j) {
+                run += L.stot[j][0];
+                crun += L.stot[j][1];
+            }
+            const uint32_t s = tid;
+            L.wcount[0][s] = c0 << 16;
+            L.wcount[1][s] = c01 | (c012 << 16);
+            L.count[s] = c;
             const uint32_t cs = crun + vc - nc;
             L.start[s] = run + v - c;
             L.cstart[s] = (uint16_t)cs;
             for (uint32_t j = 0; j < nc; ++j) L.chunk_slot[cs + j] = (uint8_t)s;
-            run += (uint32_t)__shfl((int)v, 63);
-            crun += (uint32_t)__shfl((int)vc, 63);
+            if (tid == 0u) {
+                uint32_t all = 0u;
+                for (uint32_t j = 0; j < (uint32_t)H / 64u; ++j) all += L.stot[j][1];
+                L.n_chunks = all;
+            }
         }
-        if (tid == 0u) L.n_chunks = crun;
     }
     __syncthreads();
 #pragma unroll
