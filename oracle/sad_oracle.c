@@ -521,6 +521,308 @@ ORC_API void orc_render_backward(const uint32_t* cand0, const uint32_t* cand1, c
                        width, height, cand_w, cand_h, acc, abs_acc, NULL, NULL, NULL, 0.0f);
 }
 
+/* ------------------------------------------------ exact tile-ordered sums --
+ * The HIP library sums every per-(pixel, candidate) contribution of a 16x16
+ * tile into one float partial per (tile, site, value), in a fixed order, and
+ * adds the partials of different tiles as exact fixed-point integers.  The
+ * functions below form the same partials in the same order, so the library's
+ * sums are reproduced bit for bit, and the order of the reference's float
+ * atomics (gradients.cu:422-459, utils.cu:106-231), which no run repeats, is
+ * replaced by this one.  The fp64 per-site sums above remain the independent
+ * check of these: both are sums of the same per-pair terms.
+ *
+ * Order of one (tile, site) sum: the site's entries (pixel p of the tile in
+ * raster order, candidate rank k) sorted by (p / 64, k, p % 64) - the wave,
+ * then the rank, then the lane - cut into chunks of ORC_CHUNK entries; in a
+ * chunk, lane l (of ORC_LANES) keeps a running float sum from 0 of entries
+ * l, l + 4, ...; the chunk's value is (s0 + s1) + (s2 + s3); the site's value
+ * is the running float sum from 0 of its chunks in order.
+ *
+ * Fixed point: a partial v becomes hi = trunc(v), lo = round-half-even of
+ * (v - trunc(v)) * 2^32 (|v| saturated at 2^62, NaN taken as 0); sums of hi and
+ * of lo are int64; the value is (float)((double)hi + (double)lo * 2^-32). */
+#define ORC_CHUNK 24
+#define ORC_LANES 4
+#define ORC_TILE_ENTRIES (256 * SAD_K)
+
+ORC_API void orc_fix_split(float v, int64_t* lo, int64_t* hi)
+{
+    if (isnan(v)) v = 0.0f;
+    if (v > 0x1p62f) v = 0x1p62f;
+    if (v < -0x1p62f) v = -0x1p62f;
+    float t = truncf(v);
+    *hi = (int64_t)t;
+    *lo = (int64_t)nearbyintf((v - t) * 0x1p32f);
+}
+
+ORC_API float orc_fix_value(int64_t lo, int64_t hi) { return (float)((double)hi + (double)lo * 0x1p-32); }
+
+static void orc_fix_add(int64_t* pair, float v)
+{
+    int64_t lo, hi;
+    orc_fix_split(v, &lo, &hi);
+    if (lo) __atomic_fetch_add(&pair[0], lo, __ATOMIC_RELAXED);
+    if (hi) __atomic_fetch_add(&pair[1], hi, __ATOMIC_RELAXED);
+}
+
+/* fixed-point pairs [count][2] (lo, hi) -> floats */
+ORC_API void orc_fix_values(const int64_t* pairs, float* out, int64_t count)
+{
+    for (int64_t i = 0; i < count; ++i) out[i] = orc_fix_value(pairs[2 * i], pairs[2 * i + 1]);
+}
+
+typedef struct OrcTilePixel {
+    uint32_t ids[SAD_K];
+    float w[SAD_K];
+    float pred[3], g[3], tgt[3];
+    float err;
+    uint8_t use;               /* bit k: the pair (pixel, k) is summed */
+} OrcTilePixel;
+
+/* mode 0: loss gradient (mask, 2(pred - tgt) + optional SSIM term, optional removal);
+ * mode 1: upstream grad_output (no mask; a +inf max logit drops the pixel, sad_bindings.cu:72-74);
+ * mode 2: densify statistics */
+typedef struct OrcTileJob {
+    int mode, removal;
+    const uint32_t *cand0, *cand1;
+    const float *target, *mask, *grad_out, *rendered;
+    float ssim_weight;
+    const SadSite* sites;
+    float iss;
+    uint32_t n;
+    int width, height, cand_w, cand_h;
+    int64_t* out;              /* [n][stride] (lo, hi) pairs of value j at 2 j */
+    int stride;
+    int64_t* rout;             /* removal: [n][2] */
+} OrcTileJob;
+
+/* the pixels of one tile: forward weights, dL/dpred, the summed pairs */
+static void orc_tile_pixels(const OrcTileJob* J, int tx0, int ty0, OrcTilePixel* px)
+{
+    for (int p = 0; p < 256; ++p) {
+        OrcTilePixel* q = &px[p];
+        memset(q, 0, sizeof(*q));
+        const int x = tx0 + (p & 15), y = ty0 + (p >> 4);
+        if (x >= J->width || y >= J->height) continue;
+        const size_t pix = (size_t)y * J->width + x;
+        if (J->mode != 1 && J->mask[pix] <= 0.0f) continue;
+        orc_load_ids(J->cand0, J->cand1, sad_cand_index(x, y, J->width, J->height, J->cand_w, J->cand_h), q->ids);
+        const float ux = (float)x, uy = (float)y;
+        if (J->mode == 1) {
+            float mx = -INFINITY;
+            int any = 0;
+            for (int i = 0; i < SAD_K; ++i) {
+                uint32_t id = q->ids[i];
+                if (id >= J->n || J->sites[id].pos_x < 0.0f) continue;
+                const SadSite* s = &J->sites[id];
+                float l = -sad_expf(s->log_tau) * sad_dmix2(s->pos_x, s->pos_y, s->dir_x, s->dir_y, s->log_aniso,
+                                                            s->radius, ux, uy, J->iss);
+                mx = fmaxf(mx, l);
+                any = 1;
+            }
+            if (!any || isinf(mx)) continue;
+        }
+        if (!orc_forward_px(q->ids, J->sites, J->n, ux, uy, J->iss, q->w, q->pred)) continue;
+        if (J->mode == 1) {
+            q->g[0] = J->grad_out[3 * pix]; q->g[1] = J->grad_out[3 * pix + 1]; q->g[2] = J->grad_out[3 * pix + 2];
+        } else {
+            q->tgt[0] = J->target[3 * pix]; q->tgt[1] = J->target[3 * pix + 1]; q->tgt[2] = J->target[3 * pix + 2];
+            float d0 = q->pred[0] - q->tgt[0], d1 = q->pred[1] - q->tgt[1], d2 = q->pred[2] - q->tgt[2];
+            q->g[0] = 2.0f * d0; q->g[1] = 2.0f * d1; q->g[2] = 2.0f * d2;
+            q->err = d0 * d0 + d1 * d1 + d2 * d2;
+            if (J->mode == 0 && J->ssim_weight > 0.0f) {
+                SadSsimStats st = sad_ssim_window(J->rendered, J->target, J->width, J->height, x, y);
+                float f = -J->ssim_weight * sad_ssim_dcentre(st, sad_luma3(J->rendered + 3 * pix), sad_luma3(q->tgt));
+                q->g[0] += f * 0.299f;
+                q->g[1] += f * 0.587f;
+                q->g[2] += f * 0.114f;
+            }
+        }
+        for (int k = 0; k < SAD_K; ++k) {
+            uint32_t id = q->ids[k];
+            if (id < J->n && J->sites[id].pos_x >= 0.0f) q->use |= (uint8_t)(1u << k);
+        }
+    }
+}
+
+/* the values one entry adds to its lane's running sums (nv of them); returns 0 when it adds none */
+static int orc_entry_values(const OrcTileJob* J, const OrcTilePixel* px, int p, int k, int first, int tx0, int ty0,
+                            float* v)
+{
+    const OrcTilePixel* q = &px[p];
+    const uint32_t id = q->ids[k];
+    const SadSite* s = &J->sites[id];
+    const float ex = (float)(tx0 + (p & 15)), ey = (float)(ty0 + (p >> 4));
+    int any = 0;
+    float wt = 0.0f;
+    if (first && (J->mode == 2 || J->removal))
+        for (int j = 0; j < SAD_K; ++j) wt += (q->ids[j] == id && (q->use >> j & 1u)) ? q->w[j] : 0.0f;
+    if (J->mode == 2) {
+        for (int j = 0; j < 8; ++j) v[j] = 0.0f;
+        if (first && wt > 0.0f) {
+            const float er = q->err, we = wt * fminf(er, 1.0f);
+            v[0] = wt; v[1] = wt * er; v[2] = we; v[3] = we * ex; v[4] = we * ey;
+            v[5] = we * ex * ex; v[6] = we * ex * ey; v[7] = we * ey * ey;
+            any = 1;
+        }
+        return any;
+    }
+    orc_grad_candidate(s, q->w[k], q->g, q->pred, ex, ey, J->iss, v);
+    if (J->removal) {
+        v[10] = 0.0f;
+        if (first && wt > 0.0f) v[10] = orc_removal_delta(q->pred, q->tgt, s, wt);
+    }
+    return 1;
+}
+
+/* one tile: every site's partial in the library's order, added as fixed point */
+static void orc_tile(const OrcTileJob* J, int tile, int tiles_x, OrcTilePixel* px, uint32_t* order_id,
+                     uint16_t* order_ent, uint32_t* site_ids, uint32_t* site_count, uint32_t* site_start,
+                     uint32_t* hkey, uint32_t* hval)
+{
+    const int tx0 = (tile % tiles_x) * 16, ty0 = (tile / tiles_x) * 16;
+    const int nv = J->mode == 2 ? 8 : (J->removal ? 11 : 10);
+    orc_tile_pixels(J, tx0, ty0, px);
+    /* the tile's sites (any order) via a small open-addressing map */
+    const uint32_t HM = 4096;
+    for (uint32_t i = 0; i < HM; ++i) hkey[i] = SAD_EMPTY;
+    uint32_t ns = 0, ne = 0;
+    for (int wv = 0; wv < 4; ++wv)
+        for (int k = 0; k < SAD_K; ++k)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int p = wv * 64 + lane;
+                if (!(px[p].use >> k & 1u)) continue;
+                const uint32_t id = px[p].ids[k];
+                uint32_t h = (id * 2654435761u) & (HM - 1);
+                while (hkey[h] != SAD_EMPTY && hkey[h] != id) h = (h + 1) & (HM - 1);
+                if (hkey[h] == SAD_EMPTY) { hkey[h] = id; hval[h] = ns; site_ids[ns] = id; site_count[ns] = 0; ns++; }
+                order_id[ne] = hval[h];
+                order_ent[ne] = (uint16_t)(p * SAD_K + k);
+                site_count[hval[h]]++;
+                ne++;
+            }
+    /* bucket the entries per site, keeping their order */
+    uint32_t run = 0;
+    for (uint32_t u = 0; u < ns; ++u) { site_start[u] = run; run += site_count[u]; site_count[u] = 0; }
+    uint16_t* lists = (uint16_t*)(order_ent + ORC_TILE_ENTRIES);
+    for (uint32_t e = 0; e < ne; ++e) {
+        const uint32_t u = order_id[e];
+        lists[site_start[u] + site_count[u]++] = order_ent[e];
+    }
+    for (uint32_t u = 0; u < ns; ++u) {
+        const uint32_t id = site_ids[u], beg0 = site_start[u], cnt = site_count[u];
+        float total[11] = {0};
+        for (uint32_t c0 = 0; c0 < cnt; c0 += ORC_CHUNK) {
+            const uint32_t cend = c0 + ORC_CHUNK < cnt ? c0 + ORC_CHUNK : cnt;
+            float lane[ORC_LANES][11];
+            memset(lane, 0, sizeof(lane));
+            for (int l = 0; l < ORC_LANES; ++l) {
+                for (uint32_t e = c0 + (uint32_t)l; e < cend; e += ORC_LANES) {
+                    const int ent = lists[beg0 + e];
+                    const int p = ent / SAD_K, k = ent % SAD_K;
+                    int first = 1;
+                    for (int j = 0; j < k; ++j)
+                        if (px[p].ids[j] == id && (px[p].use >> j & 1u)) first = 0;
+                    float v[11];
+                    if (!orc_entry_values(J, px, p, k, first, tx0, ty0, v)) continue;
+                    for (int j = 0; j < nv; ++j) lane[l][j] += v[j];
+                }
+            }
+            for (int j = 0; j < nv; ++j) total[j] += (lane[0][j] + lane[1][j]) + (lane[2][j] + lane[3][j]);
+        }
+        for (int j = 0; j < nv; ++j) {
+            if (J->removal && j == 10) orc_fix_add(&J->rout[2 * (size_t)id], total[j]);
+            else orc_fix_add(&J->out[(size_t)id * J->stride + 2 * j], total[j]);
+        }
+    }
+}
+
+static void orc_tiles_run(const OrcTileJob* J)
+{
+    const int tiles_x = (J->width + 15) / 16, tiles_y = (J->height + 15) / 16;
+    const int ntiles = tiles_x * tiles_y;
+    #pragma omp parallel
+    {
+        OrcTilePixel* px = (OrcTilePixel*)malloc(sizeof(OrcTilePixel) * 256);
+        uint32_t* order_id = (uint32_t*)malloc(sizeof(uint32_t) * ORC_TILE_ENTRIES);
+        uint16_t* order_ent = (uint16_t*)malloc(sizeof(uint16_t) * 2 * ORC_TILE_ENTRIES);
+        uint32_t* site_ids = (uint32_t*)malloc(sizeof(uint32_t) * ORC_TILE_ENTRIES);
+        uint32_t* site_count = (uint32_t*)malloc(sizeof(uint32_t) * ORC_TILE_ENTRIES);
+        uint32_t* site_start = (uint32_t*)malloc(sizeof(uint32_t) * ORC_TILE_ENTRIES);
+        uint32_t* hkey = (uint32_t*)malloc(sizeof(uint32_t) * 4096);
+        uint32_t* hval = (uint32_t*)malloc(sizeof(uint32_t) * 4096);
+        #pragma omp for schedule(dynamic, 4)
+        for (int t = 0; t < ntiles; ++t)
+            orc_tile(J, t, tiles_x, px, order_id, order_ent, site_ids, site_count, site_start, hkey, hval);
+        free(px); free(order_id); free(order_ent); free(site_ids); free(site_count); free(site_start);
+        free(hkey); free(hval);
+    }
+}
+
+/* computeGradientsTiledKernel (gradients.cu:103-486) with the library's sums:
+ * gfix [n][20] receives the (lo, hi) pairs of the 10 values in Site field order,
+ * rfix [n][2] (when compute_removal) the removal deltas; both are added to. */
+ORC_API void orc_gradients_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                               const SadSite* sites, float iss, uint32_t n, int compute_removal, int width,
+                               int height, int cand_w, int cand_h, const float* rendered, float ssim_weight,
+                               int64_t* gfix, int64_t* rfix)
+{
+    OrcTileJob J = {0, compute_removal ? 1 : 0, cand0, cand1, target, mask, NULL, rendered,
+                    ssim_weight > 0.0f ? ssim_weight : 0.0f, sites, iss, n, width, height, cand_w, cand_h,
+                    gfix, 20, rfix};
+    orc_tiles_run(&J);
+}
+
+/* render_sad_backward_kernel (sad_bindings.cu:100-207) with the library's sums, into gfix [n][20] */
+ORC_API void orc_render_backward_fix(const uint32_t* cand0, const uint32_t* cand1, const float* grad_out,
+                                     const SadSite* sites, float iss, uint32_t n, int width, int height,
+                                     int cand_w, int cand_h, int64_t* gfix)
+{
+    OrcTileJob J = {1, 0, cand0, cand1, NULL, NULL, grad_out, NULL, 0.0f, sites, iss, n, width, height, cand_w,
+                    cand_h, gfix, 20, NULL};
+    orc_tiles_run(&J);
+}
+
+/* computeSiteStatsSimpleKernel (utils.cu:106-231) with the library's sums, into sfix [n][16]:
+ * (lo, hi) of mass, energy, w.err, w.err.x, w.err.y, w.err.xx, w.err.xy, w.err.yy */
+ORC_API void orc_site_stats_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                                const SadSite* sites, float iss, uint32_t n, int width, int height, int cand_w,
+                                int cand_h, int64_t* sfix)
+{
+    OrcTileJob J = {2, 0, cand0, cand1, target, mask, NULL, NULL, 0.0f, sites, iss, n, width, height, cand_w, cand_h,
+                    sfix, 16, NULL};
+    orc_tiles_run(&J);
+}
+
+/* computePSNRKernel (utils.cu:5-26) with the library's block sums: per block of
+ * 256 pixels (index order) a shuffle-down tree per 64-pixel wave, the waves
+ * combined as (w0 + w1) + (w2 + w3), the blocks added as fixed point. */
+ORC_API float orc_sse_fix(const float* rendered, const float* target, const float* mask, int width, int height)
+{
+    const size_t np = (size_t)width * height;
+    int64_t acc[2] = {0, 0};
+    for (size_t b0 = 0; b0 < np; b0 += 256) {
+        float wsum[4];
+        for (int w = 0; w < 4; ++w) {
+            float v[64];
+            for (int l = 0; l < 64; ++l) {
+                const size_t i = b0 + (size_t)w * 64 + l;
+                v[l] = 0.0f;
+                if (i < np && mask[i] > 0.0f) {
+                    float d0 = rendered[3 * i] - target[3 * i], d1 = rendered[3 * i + 1] - target[3 * i + 1];
+                    float d2 = rendered[3 * i + 2] - target[3 * i + 2];
+                    v[l] = d0 * d0 + d1 * d1 + d2 * d2;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1)
+                for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+            wsum[w] = v[0];
+        }
+        orc_fix_add(acc, (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
+    }
+    return orc_fix_value(acc[0], acc[1]);
+}
+
 /* tau_diffuse_kernel, tau_diffusion.cu:27-89 */
 ORC_API void orc_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const SadSite* sites,
                              const float* graw, const float* gin, float* gout, uint32_t n,
@@ -551,11 +853,12 @@ ORC_API void orc_tau_diffuse(const uint32_t* cand0, const uint32_t* cand1, const
 }
 
 /* adamUpdateKernel, adam.cu:5-180.  b1c = 1/(1-beta1^t), b2c = 1/(1-beta2^t) are
- * computed once by the caller with libm powf (the HIP launcher does the same). */
+ * computed once per step with sad_powi_f, an integer power in double (the HIP
+ * launcher does the same), in place of the reference's powf. */
 ORC_API void orc_adam_bias(float beta1, float beta2, uint32_t t, float* b1c, float* b2c)
 {
-    *b1c = 1.0f / (1.0f - powf(beta1, (float)t));
-    *b2c = 1.0f / (1.0f - powf(beta2, (float)t));
+    *b1c = 1.0f / (1.0f - sad_powi_f(beta1, t));
+    *b2c = 1.0f / (1.0f - sad_powi_f(beta2, t));
 }
 
 ORC_API void orc_adam(SadSite* sites, SadAdamState* st, float* const g[10],
@@ -723,13 +1026,14 @@ ORC_API void orc_split_sites(SadSite* sites, SadAdamState* st, const uint32_t* i
             float eyy = ewyy[pi] / w - my * my;
             exx = fmaxf(exx, 1e-4f);
             eyy = fmaxf(eyy, 1e-4f);
-            float th = 0.5f * atan2f(2.0f * exy, exx - eyy);
-            ax = cosf(th); ay = sinf(th);
+            /* the library's own atan2 and sincos: CUDA's libdevice is not available to pin against */
+            float th = 0.5f * sad_atan2f(2.0f * exy, exx - eyy);
+            sad_sincosf(th, &ay, &ax);
             float tr = exx + eyy;
             float disc = sqrtf(fmaxf(0.0f, 0.25f * (exx - eyy) * (exx - eyy) + exy * exy));
             float l1 = fmaxf(1e-4f, 0.5f * tr + disc);
             float l2 = fmaxf(1e-4f, 0.5f * tr - disc);
-            la = sad_clampf(-0.5f * logf(l1 / l2), -2.0f, 2.0f);
+            la = sad_clampf(-0.5f * sad_logf(l1 / l2), -2.0f, 2.0f);
         } else {
             int ix = sad_clampi((int)parent.pos_x, 0, width - 1);
             int iy = sad_clampi((int)parent.pos_y, 0, height - 1);

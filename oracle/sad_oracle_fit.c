@@ -8,7 +8,13 @@
  * semantics: the gradient pass of an update iteration reads the field from
  * before the update (unless the iteration densifies, in which case the update is
  * synchronous), tau diffusion and everything after it read the updated field
- * (sad_cuda.cu:2171-2230, 2342).  solveTargetBpp / simulateFinalSites
+ * (sad_cuda.cu:2171-2230, 2342); with several passes the gradient still reads
+ * the field from before the update (the reference's side stream would be
+ * racing its own passes there).  Gradients, removal deltas, densify
+ * statistics and the logged squared error are summed as the HIP library sums
+ * them (the tile-ordered fixed-point sums of sad_oracle.c), so a fit here and
+ * the library's fit on the same inputs end with the same bits.
+ * solveTargetBpp / simulateFinalSites
  * (sad_cuda.cu:904-1066) and planSiteCapacity (sad_cuda.cu:1079-1098) are
  * restated for the site budget.
  */
@@ -30,8 +36,6 @@ void orc_pack_sites(const SadSite*, SadPackedSite*, uint32_t);
 void orc_update_candidates(const uint32_t*, const uint32_t*, uint32_t*, uint32_t*, const SadPackedSite*,
                            uint32_t, uint32_t, float, uint32_t, float, uint32_t, uint32_t, const uint32_t*,
                            const uint32_t*, uint32_t, uint32_t, int, int, int, int, int);
-void orc_gradients(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float,
-                   uint32_t, int, int, int, int, int, double*, double*, double*, double*);
 void orc_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const float*, const float*, float*,
                      uint32_t, float, int, int, int);
 void orc_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float,
@@ -48,12 +52,13 @@ void orc_split_sites(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const f
                      const float*, int, int);
 void orc_hilbert_pairs(const SadSite*, SadPair*, uint32_t, uint32_t, int, int, int);
 void orc_hilbert_order(const SadPair*, uint32_t*, uint32_t*, uint32_t);
-void orc_site_stats(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float,
-                    uint32_t, int, int, int, int, double*);
-double orc_sse(const float*, const float*, const float*, int, int);
+void orc_gradients_fix(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
+                       int, int, int, int, int, const float*, float, int64_t*, int64_t*);
+void orc_site_stats_fix(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
+                        int, int, int, int, int64_t*);
+float orc_sse_fix(const float*, const float*, const float*, int, int);
+float orc_fix_value(int64_t, int64_t);
 double orc_ssim_sum(const float*, const float*, const float*, int, int, float*);
-void orc_gradients_ssim(const uint32_t*, const uint32_t*, const float*, const float*, const float*, const SadSite*,
-                        float, uint32_t, int, float, int, int, int, int, double*, double*, double*, double*);
 
 /* Option block: the fields of TrainingOptions (sad_cuda.cu:528-576) that the
  * fit schedule reads.  init_mode 0 = sites given by the caller, 1 = per-pixel,
@@ -244,15 +249,17 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     float* tau_raw = (float*)calloc(cap, sizeof(float));
     float* tau_tmp = (float*)calloc(cap, sizeof(float));
     float* removal = (float*)calloc(cap, sizeof(float));
-    double* acc = (double*)malloc(sizeof(double) * 10 * cap);
-    double* racc = (double*)malloc(sizeof(double) * cap);
-    double* stats = (double*)malloc(sizeof(double) * 8 * cap);
+    int64_t* gfix = (int64_t*)malloc(sizeof(int64_t) * 20 * (size_t)cap);
+    int64_t* rfix = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)cap);
+    int64_t* sfix = (int64_t*)malloc(sizeof(int64_t) * 16 * (size_t)cap);
     float* st[8];
     for (int j = 0; j < 8; ++j) st[j] = (float*)calloc(cap, sizeof(float));
     SadPair* pairs = pairs_count ? (SadPair*)malloc(sizeof(SadPair) * pairs_count) : NULL;
     uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * 65536);
     uint32_t* cA0 = (uint32_t*)malloc(cells * 16), *cA1 = (uint32_t*)malloc(cells * 16);
     uint32_t* cB0 = (uint32_t*)malloc(cells * 16), *cB1 = (uint32_t*)malloc(cells * 16);
+    /* the field the gradient reads while a several-pass update runs */
+    uint32_t* cP0 = (uint32_t*)malloc(cells * 16), *cP1 = (uint32_t*)malloc(cells * 16);
     SadPackedSite* packed = (SadPackedSite*)malloc(sizeof(SadPackedSite) * cap);
     int use_hil = cfg->cand_hilbert_probes > 0 && cfg->cand_hilbert_window > 0;
     uint32_t hil_padded = ((uint32_t)imax(1, cap) + 1023u) / 1024u * 1024u;
@@ -322,6 +329,12 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
                 hil_count = n;
             }
             orc_pack_sites(sites, packed, n);
+            if (passes > 1 && !densify) {
+                memcpy(cP0, cA0, cells * 16);
+                memcpy(cP1, cA1, cells * 16);
+                gA0 = cP0;
+                gA1 = cP1;
+            }
             for (int p = 0; p < passes; ++p) {
                 uint32_t step = sad_pack_jump_step(jump_index, cw, ch);
                 orc_update_candidates(cA0, cA1, cB0, cB1, packed, n, step, iss, jump_index >> 16,
@@ -336,9 +349,10 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
         }
         if (densify && cfg->densify_enabled && needs_pairs) {
             n_dens++;
-            memset(stats, 0, sizeof(double) * 8 * n);
-            orc_site_stats(cA0, cA1, target, mask, sites, iss, n, width, height, cw, ch, stats);
-            for (uint32_t i = 0; i < n; ++i) for (int j = 0; j < 8; ++j) st[j][i] = (float)stats[8 * (size_t)i + j];
+            memset(sfix, 0, sizeof(int64_t) * 16 * n);
+            orc_site_stats_fix(cA0, cA1, target, mask, sites, iss, n, width, height, cw, ch, sfix);
+            for (uint32_t i = 0; i < n; ++i)
+                for (int j = 0; j < 8; ++j) st[j][i] = orc_fix_value(sfix[16 * (size_t)i + 2 * j], sfix[16 * (size_t)i + 2 * j + 1]);
             orc_densify_pairs(sites, st[0], st[1], pairs, n, 1.0f, cfg->densify_score_alpha, (uint32_t)pairs_count);
             orc_radix_sort(pairs, (uint32_t)pairs_count, 0xFFFFFFFFu);
             if (desired > 0 && actual < cap) {
@@ -357,20 +371,18 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
                     (iter % imax(1, cfg->prune_freq) == 0) && needs_pairs && needs_prune;
         if (prune) memset(removal, 0, sizeof(float) * n);
 
-        memset(acc, 0, sizeof(double) * 10 * n);
-        if (prune) memset(racc, 0, sizeof(double) * n);
+        memset(gfix, 0, sizeof(int64_t) * 20 * n);
+        if (prune) memset(rfix, 0, sizeof(int64_t) * 2 * n);
         if (cfg->ssim_weight > 0.0f) {
             /* sad_cuda.cu:2307-2330: render the current state, then the gradient reads it */
             orc_render(gA0, gA1, render, sites, iss, n, width, height, cw, ch);
-            orc_gradients_ssim(gA0, gA1, target, render, mask, sites, iss, n, prune, cfg->ssim_weight, width,
-                               height, cw, ch, acc, NULL, prune ? racc : NULL, NULL);
-        } else {
-            orc_gradients(gA0, gA1, target, mask, sites, iss, n, prune, width, height, cw, ch,
-                          acc, NULL, prune ? racc : NULL, NULL);
         }
+        orc_gradients_fix(gA0, gA1, target, mask, sites, iss, n, prune, width, height, cw, ch,
+                          cfg->ssim_weight > 0.0f ? render : NULL, cfg->ssim_weight, gfix, prune ? rfix : NULL);
         for (uint32_t i = 0; i < n; ++i) {
-            for (int j = 0; j < 10; ++j) grads[j][i] += (float)acc[10 * (size_t)i + j];
-            if (prune) removal[i] += (float)racc[i];
+            for (int j = 0; j < 10; ++j)
+                grads[j][i] += orc_fix_value(gfix[20 * (size_t)i + 2 * j], gfix[20 * (size_t)i + 2 * j + 1]);
+            if (prune) removal[i] += orc_fix_value(rfix[2 * (size_t)i], rfix[2 * (size_t)i + 1]);
         }
 
         memcpy(tau_raw, grads[2], sizeof(float) * n);
@@ -402,7 +414,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
 
         if (iter % imax(1, cfg->log_freq) == 0 || iter == cfg->iterations - 1) {
             orc_render(cA0, cA1, render, sites, iss, n, width, height, cw, ch);
-            float sse = (float)orc_sse(render, target, mask, width, height);
+            float sse = orc_sse_fix(render, target, mask, width, height);
             float mse = sse / (fmaxf(1.0f, mask_sum) * 3.0f);
             float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / sqrtf(mse)) : 100.0f;
             if (log_psnr && n_logs < log_cap) log_psnr[n_logs] = psnr;
@@ -436,7 +448,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
     free(mask); free(adam);
     for (int j = 0; j < 10; ++j) free(grads[j]);
     for (int j = 0; j < 8; ++j) free(st[j]);
-    free(tau_raw); free(tau_tmp); free(removal); free(acc); free(racc); free(stats); free(pairs); free(idx);
-    free(cA0); free(cA1); free(cB0); free(cB1); free(packed); free(hpairs); free(horder); free(hpos); free(render);
+    free(tau_raw); free(tau_tmp); free(removal); free(gfix); free(rfix); free(sfix); free(pairs); free(idx);
+    free(cA0); free(cA1); free(cB0); free(cB1); free(cP0); free(cP1); free(packed); free(hpairs); free(horder); free(hpos); free(render);
     return 0;
 }

@@ -53,19 +53,19 @@ struct FitRun {
     float mask_sum = 0.0f, iss = 0.0f, delta_norm = 0.0f;
     DevArena mem;
     float *d_target = nullptr, *d_mask = nullptr, *d_render = nullptr;
-    double* d_mse = nullptr;
-    double* d_ssim = nullptr;          /* ssim_metric: sum of the per-pixel SSIM */
+    long long* d_mse = nullptr;        /* fixed-point (lo, hi) sums */
+    long long* d_ssim = nullptr;       /* ssim_metric: sum of the per-pixel SSIM */
     float ssim_weight = 0.0f;          /* SSIM loss weight */
     SadInferenceSite* d_packed = nullptr;   /* packed_psnr: quantised sites, their quant, decoded sites */
     SadInferenceQuant* d_quant = nullptr;
     SadSite* d_dec = nullptr;
     float *tau_raw = nullptr, *tau_tmp = nullptr, *tau_tmp2 = nullptr;
-    double* removal = nullptr;
+    long long* removal = nullptr;      /* fixed-point (lo, hi) pair per site */
     uint32_t* tau_nbr = nullptr;
     SadSite* d_sites = nullptr;
     SadAdamState* d_adam = nullptr;
-    double* gacc = nullptr;
-    double* st[8] = {};
+    long long* gacc = nullptr;         /* fixed-point gradient lines, FIX_GRAD_STRIDE words per site */
+    long long* st = nullptr;           /* fixed-point statistics lines, FIX_STAT_STRIDE words per site */
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count = nullptr;
     uint32_t *fa0 = nullptr, *fa1 = nullptr, *fb0 = nullptr, *fb1 = nullptr;
@@ -78,7 +78,7 @@ struct FitRun {
     bool hil_ready = false;
     float best = 0.0f, final_psnr = 0.0f;
     float final_ssim = 0.0f, best_ssim = 0.0f, packed_psnr = 0.0f;
-    double sse = 0.0, ssim_sum = 0.0;
+    long long sse[2] = {0, 0}, ssim_sum[2] = {0, 0};
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
     double submit_s = 0.0, admit_s = 0.0;
@@ -96,9 +96,10 @@ size_t fit_bytes(const SadhFitOptions& o, int width, int height, int cap, int pa
     const int ds = std::max(1, o.cand_downscale);
     const size_t cells = (size_t)((width + ds - 1) / ds) * ((height + ds - 1) / ds);
     size_t b = cells * 64 + np * (12 + 4 + 12);
-    /* site, Adam state, double gradient line, 3 tau buffers, double removal delta, tau neighbour list, search site */
-    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 128 + 12 + 8 + 32 + sizeof(SadSearchSite));
-    if (o.densify_enabled) b += (size_t)cap * 64;
+    /* site, Adam state, fixed-point gradient line, 3 tau buffers, removal pair, tau neighbour list, search site */
+    b += (size_t)cap * (sizeof(SadSite) + sizeof(SadAdamState) + 8 * FIX_GRAD_STRIDE + 12 + 16 + 32 +
+                        sizeof(SadSearchSite));
+    if (o.densify_enabled) b += (size_t)cap * 8 * FIX_STAT_STRIDE;
     if (o.packed_psnr) b += (size_t)cap * (sizeof(SadInferenceSite) + sizeof(SadSite)) + sizeof(SadInferenceQuant);
     if (pairs_count > 0) b += (size_t)pairs_count * 16 + 256 * 4 * (pairs_count / 1024 + 1) + 2 * 65536 * 4;
     if (use_hil) b += (size_t)cap * 8 + 16 * (size_t)(cap / 1024 + 1) * 1024 * 2;
@@ -157,8 +158,8 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.d_target, (size_t)J.np * 3));
     SAD_TRY(J.mem.alloc(&J.d_mask, (size_t)J.np));
     SAD_TRY(J.mem.alloc(&J.d_render, (size_t)J.np * 3));
-    SAD_TRY(J.mem.alloc(&J.d_mse, 1));
-    SAD_TRY(J.mem.alloc(&J.d_ssim, 1));
+    SAD_TRY(J.mem.alloc(&J.d_mse, 2));
+    SAD_TRY(J.mem.alloc(&J.d_ssim, 2));
     J.ssim_weight = base.ssim_weight > 0.0f ? base.ssim_weight : 0.0f;
     if (base.packed_psnr) {
         SAD_TRY(J.mem.alloc(&J.d_packed, (size_t)cap));
@@ -168,13 +169,13 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     SAD_TRY(J.mem.alloc(&J.d_count, 1));
     SAD_TRY(J.mem.alloc(&J.d_sites, cap));
     SAD_TRY(J.mem.alloc(&J.d_adam, cap));
-    SAD_TRY(J.mem.alloc(&J.gacc, (size_t)cap * 16));
+    SAD_TRY(J.mem.alloc(&J.gacc, (size_t)cap * FIX_GRAD_STRIDE));
     SAD_TRY(J.mem.alloc(&J.tau_raw, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp, cap));
     SAD_TRY(J.mem.alloc(&J.tau_tmp2, cap));
     SAD_TRY(J.mem.alloc(&J.tau_nbr, (size_t)cap * 8));
-    SAD_TRY(J.mem.alloc(&J.removal, cap));
-    if (base.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(J.mem.alloc(&J.st[j], cap));
+    SAD_TRY(J.mem.alloc(&J.removal, (size_t)cap * 2));
+    if (base.densify_enabled) SAD_TRY(J.mem.alloc(&J.st, (size_t)cap * FIX_STAT_STRIDE));
     if (J.pairs_count > 0) {
         SAD_TRY(J.mem.alloc(&J.pairs, J.pairs_count));
         SAD_TRY(J.mem.alloc(&J.sort_scratch, J.pairs_count));
@@ -199,8 +200,8 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     HIP_TRY(hipMemcpyAsync(J.d_mask, mask.data(), sizeof(float) * J.np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(J.d_sites, hs.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(J.d_adam, 0, sizeof(SadAdamState) * cap, S));
-    HIP_TRY(hipMemsetAsync(J.gacc, 0, sizeof(double) * 16 * cap, S));
-    HIP_TRY(hipMemsetAsync(J.removal, 0, sizeof(double) * cap, S));
+    HIP_TRY(hipMemsetAsync(J.gacc, 0, sizeof(long long) * FIX_GRAD_STRIDE * cap, S));
+    HIP_TRY(hipMemsetAsync(J.removal, 0, sizeof(long long) * 2 * cap, S));
     J.actual = io.n_sites;
     J.n = (uint32_t)J.actual;
     J.active_est = 0;
@@ -348,8 +349,10 @@ struct Engine {
             h_steps = nullptr;
             d_steps = nullptr;
             steps_cap = std::max(nj, 2 * steps_cap);
-            HIP_TRY(hipHostMalloc((void**)&h_steps, sizeof(JobStep) * RING * steps_cap, hipHostMallocDefault));
-            HIP_TRY(hipMalloc((void**)&d_steps, sizeof(JobStep) * RING * steps_cap));
+            /* per ring slot: the tick's table, then a copy whose vpt flags select the updates that run after
+             * the gradient (multi-pass updates of fits that do not densify) */
+            HIP_TRY(hipHostMalloc((void**)&h_steps, sizeof(JobStep) * RING * 2 * steps_cap, hipHostMallocDefault));
+            HIP_TRY(hipMalloc((void**)&d_steps, sizeof(JobStep) * RING * 2 * steps_cap));
         }
         if (nj > 0) HIP_TRY(hipMemcpy(d_jobs, desc.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice));
         table_dirty = false;
@@ -430,8 +433,12 @@ struct Engine {
         /* per-fit flags and step table (a ring slot whose previous tick has finished) */
         const int slot = (int)(tick_no % RING);
         HIP_TRY(hipEventSynchronize(slot_done[slot]));
-        JobStep* hs = h_steps + (size_t)slot * steps_cap;
-        JobStep* ds = d_steps + (size_t)slot * steps_cap;
+        JobStep* hs = h_steps + (size_t)slot * 2 * steps_cap;
+        JobStep* ds = d_steps + (size_t)slot * 2 * steps_cap;
+        /* An update of several passes ping-pongs through the field its fit's gradient reads, so it never runs
+         * beside that gradient: a densifying fit's update runs before its gradient (which reads the new
+         * field), the others' after it. */
+        const bool split_upd = std::max(1, o.cand_update_passes) > 1;
         bool any_upd = false, any_dens_upd = false, any_prune = false, any_log = false, any_dens = false;
         for (int j = 0; j < nj; ++j) {
             FitRun& J = *fits[j];
@@ -452,12 +459,16 @@ struct Engine {
             const float blend = (float)it / (float)std::max(1, J.iterations);
             const uint32_t t = (uint32_t)(it + 1);
             hs[j].lambda = 0.05f * (0.1f + 0.9f * blend);
-            hs[j].b1c = 1.0f / (1.0f - powf(o.beta1, (float)t));
-            hs[j].b2c = 1.0f / (1.0f - powf(o.beta2, (float)t));
+            hs[j].b1c = 1.0f / (1.0f - sad_powi_f(o.beta1, t));
+            hs[j].b2c = 1.0f / (1.0f - sad_powi_f(o.beta2, t));
             hs[j].jump_index = J.jump_index;
-            hs[j].vpt = J.upd ? 1u : 0u;
+            hs[j].vpt = (J.upd && (!split_upd || J.dens)) ? 1u : 0u;
         }
-        HIP_TRY(hipMemcpyAsync(ds, hs, sizeof(JobStep) * nj, hipMemcpyHostToDevice, S));
+        for (int j = 0; j < nj; ++j) {
+            hs[nj + j] = hs[j];
+            hs[nj + j].vpt = (split_upd && fits[j]->upd && !fits[j]->dens) ? 1u : 0u;
+        }
+        HIP_TRY(hipMemcpyAsync(ds, hs, sizeof(JobStep) * 2 * nj, hipMemcpyHostToDevice, S));
         if (any_upd) HIP_TRY(hipEventRecord(steps_ready, S));
 
         const bool steady = o.graph_ticks && !any_upd && !any_dens && !any_prune && !any_log && !o.profile_gradients;
@@ -470,17 +481,36 @@ struct Engine {
         return finish_tick(any_log, done);
     }
 
+    /* the VPT passes of this tick's update for the jobs whose flag is set in `steps`, on stream U */
+    int vpt_passes(int nj, const JobStep* steps, hipStream_t U)
+    {
+        const int passes = std::max(1, o.cand_update_passes);
+        for (int p = 0; p < passes; ++p) {
+            BatchStep stp = {};
+            stp.vpt_pass = (uint32_t)p;
+            stp.radius_scale = o.cand_radius_scale;
+            stp.radius_probes = o.cand_radius_probes;
+            stp.inject_count = o.cand_inject_count;
+            stp.hil_probes = use_hil ? o.cand_hilbert_probes : 0u;
+            stp.hil_window = use_hil ? o.cand_hilbert_window : 0u;
+            SAD_TRY(launch_vpt_batch(d_jobs, (uint32_t)nj, steps, total_ctiles, total_sites, stp, U));
+        }
+        return SADH_OK;
+    }
+
     /* a tick's launches when some fit updates candidates, densifies, prunes or logs (or graphs are off) */
     int eager_tick(int nj, const JobStep* ds, bool any_upd, bool any_dens_upd, bool any_prune)
     {
         hipStream_t S = sp.main;
         const int passes = std::max(1, o.cand_update_passes);
+        const bool split_upd = passes > 1;
         std::vector<uint32_t> late;
-        bool pending = false;
+        bool pending = false, deferred = false;
         if (any_upd) {
-            /* an updating fit that densifies needs its new field first: then all updates run on S */
-            hipStream_t U = any_dens_upd ? S : sp.cand;
-            if (!any_dens_upd) HIP_TRY(hipStreamWaitEvent(sp.cand, steps_ready, 0));
+            /* an updating fit that densifies needs its new field first: then all single-pass updates run on
+             * S; multi-pass updates run on S as well, those of non-densifying fits after the gradient */
+            hipStream_t U = (any_dens_upd || split_upd) ? S : sp.cand;
+            if (U != S) HIP_TRY(hipStreamWaitEvent(sp.cand, steps_ready, 0));
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.upd) continue;
@@ -496,16 +526,7 @@ struct Engine {
                     J.hil_count = J.n;
                 }
             }
-            for (int p = 0; p < passes; ++p) {
-                BatchStep stp = {};
-                stp.vpt_pass = (uint32_t)p;
-                stp.radius_scale = o.cand_radius_scale;
-                stp.radius_probes = o.cand_radius_probes;
-                stp.inject_count = o.cand_inject_count;
-                stp.hil_probes = use_hil ? o.cand_hilbert_probes : 0u;
-                stp.hil_window = use_hil ? o.cand_hilbert_window : 0u;
-                SAD_TRY(launch_vpt_batch(d_jobs, (uint32_t)nj, ds, total_ctiles, total_sites, stp, U));
-            }
+            SAD_TRY(vpt_passes(nj, ds, U));
             /* an odd pass count leaves the result in the other field: densifying fits switch now,
              * the others after their gradient, which reads the field from before the update */
             std::vector<uint32_t> now_flip;
@@ -514,11 +535,12 @@ struct Engine {
                 if (!J.upd) continue;
                 J.jump_index += (uint32_t)passes;
                 if (passes & 1) (J.dens ? now_flip : late).push_back((uint32_t)j);
+                if (split_upd && !J.dens) deferred = true;
             }
             SAD_TRY(flip(now_flip, S));
             for (uint32_t j : now_flip) fits[j]->cur ^= 1u;
             if (o.ssim_weight > 0.0f && (!now_flip.empty() || !late.empty())) epoch++;
-            if (!any_dens_upd) {
+            if (U != S) {
                 HIP_TRY(hipEventRecord(sp.cand_done, sp.cand));
                 pending = true;
             }
@@ -530,18 +552,18 @@ struct Engine {
             if (!J.dens || J.pairs_count <= 0) continue;
             J.n_dens++;
             const int desired = std::max(0, (int)((float)J.active_est * J.d_pct));
-            SAD_TRY(launch_clear64(J.st, 8, J.n, S));
-            SAD_TRY(launch_site_stats64(J.c0(), J.c1(), J.d_target, J.d_mask, J.d_sites, J.iss, J.n, J.st, J.width,
-                                      J.height, J.cw, J.ch, S));
-            SAD_TRY(launch_densify_pairs(J.d_sites, J.st[0], J.st[1], J.pairs, J.n, 1.0f, o.densify_score_alpha,
-                                         (uint32_t)J.pairs_count, S));
+            SAD_TRY(launch_clear_fix(J.st, (size_t)J.n * FIX_STAT_STRIDE, S));
+            SAD_TRY(launch_site_stats_fix(J.c0(), J.c1(), J.d_target, J.d_mask, J.d_sites, J.iss, J.n, J.st, J.width,
+                                          J.height, J.cw, J.ch, S));
+            SAD_TRY(launch_densify_pairs_fix(J.d_sites, J.st, J.pairs, J.n, 1.0f, o.densify_score_alpha,
+                                             (uint32_t)J.pairs_count, S));
             SAD_TRY(launch_radix_sort(J.pairs, J.sort_scratch, J.sort_hist, std::min(J.n, (uint32_t)J.pairs_count),
                                       0xFFFFFFFFu, S));
             int k = std::min(desired, std::min(J.cap - J.actual, 65536));
             if (k > 0) {
                 SAD_TRY(launch_write_indices(J.pairs, J.split_idx, (uint32_t)k, S));
-                SAD_TRY(launch_split(J.d_sites, J.d_adam, J.split_idx, (uint32_t)k, J.st, (uint32_t)J.actual,
-                                     J.d_target, J.width, J.height, S));
+                SAD_TRY(launch_split_fix(J.d_sites, J.d_adam, J.split_idx, (uint32_t)k, J.st, (uint32_t)J.actual,
+                                         J.d_target, J.width, J.height, S));
                 J.actual += k;
                 J.n = (uint32_t)J.actual;
                 J.active_est += k;
@@ -556,7 +578,7 @@ struct Engine {
             epoch++;
         }
         if (any_prune)
-            for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear64(&fits[j]->removal, 1, fits[j]->n, S));
+            for (int j = 0; j < nj; ++j) SAD_TRY(launch_clear_fix(fits[j]->removal, 2u * (size_t)fits[j]->n, S));
 
         if (o.profile_gradients) {
             hipEvent_t a, b;
@@ -582,6 +604,7 @@ struct Engine {
             gbytes.push_back(b);
         }
         if (pending) HIP_TRY(hipStreamWaitEvent(S, sp.cand_done, 0));
+        if (deferred) SAD_TRY(vpt_passes(nj, ds + nj, S));
         SAD_TRY(flip(late, S));
         for (uint32_t j : late) fits[j]->cur ^= 1u;
 
@@ -591,7 +614,8 @@ struct Engine {
             FitRun& J = *fits[j];
             if (!J.prune) continue;
             J.n_prune++;
-            SAD_TRY(launch_prune_pairs(J.d_sites, J.removal, J.pairs, J.n, J.delta_norm, (uint32_t)J.pairs_count, S));
+            SAD_TRY(launch_prune_pairs_fix(J.d_sites, J.removal, J.pairs, J.n, J.delta_norm, (uint32_t)J.pairs_count,
+                                           S));
             SAD_TRY(launch_radix_sort(J.pairs, J.sort_scratch, J.sort_hist, std::min(J.n, (uint32_t)J.pairs_count),
                                       0xFFFFFFFFu, S));
             int k = std::min(std::max(0, (int)((float)J.active_est * J.p_pct)), 65536);
@@ -615,24 +639,24 @@ struct Engine {
                 if (!J.log) continue;
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
                                       J.ch, S));
-                SAD_TRY(launch_sse64(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
+                SAD_TRY(launch_sse_fix(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
                 if (o.ssim_metric) {
-                    SAD_TRY(launch_ssim64(J.d_render, J.d_target, J.d_mask, J.d_ssim, J.width, J.height, S));
-                    HIP_TRY(hipMemcpyAsync(&J.ssim_sum, J.d_ssim, sizeof(double), hipMemcpyDeviceToHost, S));
+                    SAD_TRY(launch_ssim_fix(J.d_render, J.d_target, J.d_mask, J.d_ssim, J.width, J.height, S));
+                    HIP_TRY(hipMemcpyAsync(J.ssim_sum, J.d_ssim, sizeof(J.ssim_sum), hipMemcpyDeviceToHost, S));
                 }
                 SAD_TRY(launch_count_active(J.d_sites, J.d_count, J.n, S));
-                HIP_TRY(hipMemcpyAsync(&J.sse, J.d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
+                HIP_TRY(hipMemcpyAsync(J.sse, J.d_mse, sizeof(J.sse), hipMemcpyDeviceToHost, S));
                 HIP_TRY(hipMemcpyAsync(&J.active_count, J.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             }
             HIP_TRY(hipStreamSynchronize(S));
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.log) continue;
-                float mse = (float)J.sse / (std::max(1.0f, J.mask_sum) * 3.0f);
+                float mse = fix_value_host(J.sse) / (std::max(1.0f, J.mask_sum) * 3.0f);
                 float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
                 if (J.io.log_psnr && J.n_logs < J.io.log_cap) J.io.log_psnr[J.n_logs] = psnr;
                 if (o.ssim_metric) {
-                    const float ssim = (float)J.ssim_sum / std::max(1.0f, J.mask_sum);   /* sad_cuda.cu:2443-2448 */
+                    const float ssim = fix_value_host(J.ssim_sum) / std::max(1.0f, J.mask_sum);   /* sad_cuda.cu:2443-2448 */
                     SadhFitResult* r = J.io.result;
                     if (r && r->log_ssim && J.n_logs < J.io.log_cap) r->log_ssim[J.n_logs] = ssim;
                     J.final_ssim = ssim;
@@ -675,12 +699,12 @@ struct Engine {
             SAD_TRY(launch_pack_inference(J.d_sites, J.n, J.width, J.height, J.d_packed, J.d_quant, S));
             SAD_TRY(launch_unpack_inference(J.d_packed, J.d_quant, J.n, J.width, J.height, J.d_dec, S));
             SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_dec, J.iss, J.n, J.width, J.height, J.cw, J.ch, S));
-            SAD_TRY(launch_sse64(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
-            double sse = 0.0;
-            HIP_TRY(hipMemcpyAsync(&sse, J.d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
+            SAD_TRY(launch_sse_fix(J.d_render, J.d_target, J.d_mask, J.d_mse, J.width, J.height, S));
+            long long sse[2] = {0, 0};
+            HIP_TRY(hipMemcpyAsync(sse, J.d_mse, sizeof(sse), hipMemcpyDeviceToHost, S));
             if (r && r->packed_render) HIP_TRY(hipMemcpyAsync(r->packed_render, J.d_render, img, hipMemcpyDeviceToHost, S));
             HIP_TRY(hipStreamSynchronize(S));
-            const float mse = (float)sse / (std::max(1.0f, J.mask_sum) * 3.0f);
+            const float mse = fix_value_host(sse) / (std::max(1.0f, J.mask_sum) * 3.0f);
             J.packed_psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
         }
         if (r && r->final_render) {

@@ -33,12 +33,12 @@ struct BatchJob {
     SadSite* sites;
     SadAdamState* adam;
     SadSearchSite* search;
-    double* gacc;              /* gradients: AoS [cap][16] double lines, values 0-9 in Site field order */
+    long long* gacc;           /* gradients: fixed-point lines, FIX_GRAD_STRIDE words per site */
     float* tau_raw;
     float* tau_tmp;
     float* tau_tmp2;
     uint32_t* tau_nbr;         /* 8 ids per site: neighbour lists shared by the tau passes */
-    double* removal;           /* prune iterations: removal deltas */
+    long long* removal;        /* prune iterations: removal deltas, one fixed-point (lo, hi) pair per site */
     const float* rendered;     /* SSIM loss: render of the current state, written before the gradient */
     float ssim_weight;         /* SSIM loss weight (0: L2 only) */
     const uint32_t* h_order;   /* null: no Hilbert probes */

@@ -165,16 +165,16 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
 
     DevArena mem;
     float *d_target, *d_mask, *d_render, *tau_raw, *tau_tmp, *tau_tmp2;
-    double* d_mse;
+    long long* d_mse;
     SadSite* d_sites;
     SadAdamState* d_adam;
-    /* Cross-tile sums in double: gradients as AoS [cap][16] lines (one 128-byte atomic target per site),
-     * removal deltas and site statistics.  Each tile adds one float partial per site and value; summed in
-     * double and rounded once where they are read, the values do not depend on the order in which tiles
-     * finish, so a fit repeats exactly. */
-    double* gacc;
-    double* removal;
-    double* st[8] = {};
+    /* Cross-tile sums in exact fixed point: gradients as lines of FIX_GRAD_STRIDE int64 words per site,
+     * removal deltas as one (lo, hi) pair per site, site statistics as lines of FIX_STAT_STRIDE words.  Each
+     * tile adds one float partial per site and value as two integer atomics; rounded once to float where
+     * they are read, the values do not depend on the order in which tiles finish, so a fit repeats exactly. */
+    long long* gacc;
+    long long* removal;
+    long long* st = nullptr;
     SadPair *pairs = nullptr, *sort_scratch = nullptr;
     uint32_t *sort_hist = nullptr, *split_idx = nullptr, *prune_idx = nullptr, *d_count;
     uint32_t *cA0, *cA1, *cB0, *cB1;
@@ -182,20 +182,20 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&d_target, (size_t)np * 3));
     SAD_TRY(mem.alloc(&d_mask, (size_t)np));
     SAD_TRY(mem.alloc(&d_render, (size_t)np * 3));
-    SAD_TRY(mem.alloc(&d_mse, 1));
-    double* d_ssim;
-    SAD_TRY(mem.alloc(&d_ssim, 1));
+    SAD_TRY(mem.alloc(&d_mse, 2));
+    long long* d_ssim;
+    SAD_TRY(mem.alloc(&d_ssim, 2));
     SAD_TRY(mem.alloc(&d_count, 1));
     SAD_TRY(mem.alloc(&d_sites, cap));
     SAD_TRY(mem.alloc(&d_adam, cap));
-    SAD_TRY(mem.alloc(&gacc, (size_t)cap * 16));
+    SAD_TRY(mem.alloc(&gacc, (size_t)cap * FIX_GRAD_STRIDE));
     SAD_TRY(mem.alloc(&tau_raw, cap));
     SAD_TRY(mem.alloc(&tau_tmp, cap));
     SAD_TRY(mem.alloc(&tau_tmp2, cap));
     uint32_t* tau_nbr;   /* per-site neighbour lists shared by the four tau passes */
     SAD_TRY(mem.alloc(&tau_nbr, (size_t)cap * 8));
-    SAD_TRY(mem.alloc(&removal, cap));
-    if (o.densify_enabled) for (int j = 0; j < 8; ++j) SAD_TRY(mem.alloc(&st[j], cap));
+    SAD_TRY(mem.alloc(&removal, (size_t)cap * 2));
+    if (o.densify_enabled) SAD_TRY(mem.alloc(&st, (size_t)cap * FIX_STAT_STRIDE));
     if (pairs_count > 0) {
         SAD_TRY(mem.alloc(&pairs, pairs_count));
         SAD_TRY(mem.alloc(&sort_scratch, pairs_count));
@@ -207,6 +207,13 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     SAD_TRY(mem.alloc(&cA1, cells * 4));
     SAD_TRY(mem.alloc(&cB0, cells * 4));
     SAD_TRY(mem.alloc(&cB1, cells * 4));
+    /* An update of several passes on the second stream ping-pongs through a third field, so no pass writes
+     * the field the concurrent gradient reads (with two buffers an even pass count would end there). */
+    uint32_t *cC0 = nullptr, *cC1 = nullptr;
+    if (o.cand_update_passes > 1) {
+        SAD_TRY(mem.alloc(&cC0, cells * 4));
+        SAD_TRY(mem.alloc(&cC1, cells * 4));
+    }
     SAD_TRY(mem.alloc(&search, cap));
     const bool use_hil = o.cand_hilbert_probes > 0 && o.cand_hilbert_window > 0;
     const uint32_t hil_padded = div_up((uint32_t)std::max(1, cap), 1024u) * 1024u;
@@ -231,8 +238,8 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
     HIP_TRY(hipMemcpyAsync(d_mask, mask.data(), sizeof(float) * np, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemcpyAsync(d_sites, host_sites.data(), sizeof(SadSite) * cap, hipMemcpyHostToDevice, S));
     HIP_TRY(hipMemsetAsync(d_adam, 0, sizeof(SadAdamState) * cap, S));
-    HIP_TRY(hipMemsetAsync(gacc, 0, sizeof(double) * 16 * cap, S));
-    HIP_TRY(hipMemsetAsync(removal, 0, sizeof(double) * cap, S));
+    HIP_TRY(hipMemsetAsync(gacc, 0, sizeof(long long) * FIX_GRAD_STRIDE * cap, S));
+    HIP_TRY(hipMemsetAsync(removal, 0, sizeof(long long) * 2 * cap, S));
 
     int actual = n0;
     uint32_t n = (uint32_t)actual;
@@ -311,8 +318,16 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
                                           use_hil ? o.cand_hilbert_probes : 0u, use_hil ? o.cand_hilbert_window : 0u,
                                           ds, width, height, cw, ch, U));
                 jump_index++;
-                std::swap(nA0, nB0);
-                std::swap(nA1, nB1);
+                if (!densify && p == 0 && passes > 1) {
+                    /* A (read by the gradient) -> B, then B <-> C; A becomes the spare */
+                    uint32_t* a0 = nA0; uint32_t* a1 = nA1;
+                    nA0 = nB0; nA1 = nB1;
+                    nB0 = cC0; nB1 = cC1;
+                    cC0 = a0; cC1 = a1;
+                } else {
+                    std::swap(nA0, nB0);
+                    std::swap(nA1, nB1);
+                }
             }
             if (densify) {
                 cA0 = nA0; cA1 = nA1; cB0 = nB0; cB1 = nB1;
@@ -323,18 +338,18 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         }
         if (densify && o.densify_enabled && needs_pairs) {
             n_dens++;
-            SAD_TRY(launch_clear64(st, 8, n, S));
-            SAD_TRY(launch_site_stats64(cA0, cA1, d_target, d_mask, d_sites, iss, n, st, width, height, cw, ch, S));
-            SAD_TRY(launch_densify_pairs(d_sites, st[0], st[1], pairs, n, 1.0f, o.densify_score_alpha,
-                                         (uint32_t)pairs_count, S));
+            SAD_TRY(launch_clear_fix(st, (size_t)n * FIX_STAT_STRIDE, S));
+            SAD_TRY(launch_site_stats_fix(cA0, cA1, d_target, d_mask, d_sites, iss, n, st, width, height, cw, ch, S));
+            SAD_TRY(launch_densify_pairs_fix(d_sites, st, pairs, n, 1.0f, o.densify_score_alpha,
+                                             (uint32_t)pairs_count, S));
             /* pairs at index >= n hold the (0xFFFFFFFF, 0xFFFFFFFF) sentinel: the stable sort of the
              * whole buffer is the sorted first n pairs followed by them, so only those n are sorted */
             SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, std::min(n, (uint32_t)pairs_count), 0xFFFFFFFFu, S));
             int k = std::min(desired, std::min(cap - actual, 65536));
             if (k > 0) {
                 SAD_TRY(launch_write_indices(pairs, split_idx, (uint32_t)k, S));
-                SAD_TRY(launch_split(d_sites, d_adam, split_idx, (uint32_t)k, st, (uint32_t)actual, d_target, width,
-                                     height, S));
+                SAD_TRY(launch_split_fix(d_sites, d_adam, split_idx, (uint32_t)k, st, (uint32_t)actual, d_target,
+                                         width, height, S));
                 actual += k;
                 n = (uint32_t)actual;
                 active_est += k;
@@ -344,7 +359,7 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         const float lambda = 0.05f * (0.1f + 0.9f * blend);
         const bool prune = p_pct > 0.0f && iter >= eff_ps && iter < eff_pe && iter % std::max(1, o.prune_freq) == 0 &&
                            needs_pairs && needs_prune;
-        if (prune) SAD_TRY(launch_clear64(&removal, 1, n, S));
+        if (prune) SAD_TRY(launch_clear_fix(removal, 2u * (size_t)n, S));
 
         if (o.profile_gradients) {
             hipEvent_t a, b;
@@ -358,9 +373,9 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         if (o.ssim_weight > 0.0f) SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
         /* the tail's neighbour lists are written by extra blocks of the gradient launch when the tail uses the
          * gradient's field; after a candidate update on the second stream the first tau pass writes them */
-        SAD_TRY(launch_gradients_lines64(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width,
-                                         height, cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f,
-                                         pending ? nullptr : tau_nbr, lambda, ds));
+        SAD_TRY(launch_gradients_fix(cA0, cA1, d_target, d_mask, gacc, d_sites, iss, n, removal, prune, width,
+                                     height, cw, ch, S, d_render, o.ssim_weight > 0.0f ? o.ssim_weight : 0.0f,
+                                     pending ? nullptr : tau_nbr, lambda, ds));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             gbytes.push_back(48.0 * np + 120.0 * n + (prune ? 8.0 * n : 0.0));
@@ -371,13 +386,13 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
             cA0 = nA0; cA1 = nA1; cB0 = nB0; cB1 = nB1;
         }
         /* 4 tau passes (raw saved by the first), the last fused with Adam + clear of the gradient lines */
-        SAD_TRY(launch_tau_adam_lines64(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites,
-                                        d_adam, lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
-                                        (uint32_t)(iter + 1), n, width, height, S, tau_nbr, !pending));
+        SAD_TRY(launch_tau_adam_fix(cA0, cA1, gacc, tau_raw, tau_tmp, tau_tmp2, lambda, cw, ch, ds, d_sites, d_adam,
+                                    lr_pos, lr_tau, lr_rad, lr_col, lr_dir, lr_an, o.beta1, o.beta2, o.eps,
+                                    (uint32_t)(iter + 1), n, width, height, S, tau_nbr, !pending));
 
         if (prune) {
             n_prune++;
-            SAD_TRY(launch_prune_pairs(d_sites, removal, pairs, n, delta_norm, (uint32_t)pairs_count, S));
+            SAD_TRY(launch_prune_pairs_fix(d_sites, removal, pairs, n, delta_norm, (uint32_t)pairs_count, S));
             SAD_TRY(launch_radix_sort(pairs, sort_scratch, sort_hist, std::min(n, (uint32_t)pairs_count), 0xFFFFFFFFu, S));
             int k = std::min(std::max(0, (int)((float)active_est * p_pct)), 65536);
             if (k > 0) {
@@ -393,19 +408,20 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
 
         if (iter % std::max(1, o.log_freq) == 0 || iter == o.iterations - 1) {
             SAD_TRY(launch_render(cA0, cA1, d_render, d_sites, iss, n, width, height, cw, ch, S));
-            SAD_TRY(launch_sse64(d_render, d_target, d_mask, d_mse, width, height, S));
-            if (o.ssim_metric) SAD_TRY(launch_ssim64(d_render, d_target, d_mask, d_ssim, width, height, S));
+            SAD_TRY(launch_sse_fix(d_render, d_target, d_mask, d_mse, width, height, S));
+            if (o.ssim_metric) SAD_TRY(launch_ssim_fix(d_render, d_target, d_mask, d_ssim, width, height, S));
             SAD_TRY(launch_count_active(d_sites, d_count, n, S));
-            double sse = 0.0, ssim_sum = 0.0;
-            HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
-            if (o.ssim_metric) HIP_TRY(hipMemcpyAsync(&ssim_sum, d_ssim, sizeof(double), hipMemcpyDeviceToHost, S));
+            long long sse_fx[2] = {0, 0}, ssim_fx[2] = {0, 0};
+            HIP_TRY(hipMemcpyAsync(sse_fx, d_mse, sizeof(sse_fx), hipMemcpyDeviceToHost, S));
+            if (o.ssim_metric) HIP_TRY(hipMemcpyAsync(ssim_fx, d_ssim, sizeof(ssim_fx), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipMemcpyAsync(&active_count, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, S));
             HIP_TRY(hipStreamSynchronize(S));
-            float mse = (float)sse / (std::max(1.0f, mask_sum) * 3.0f);
+            const float sse = fix_value_host(sse_fx), ssim_sum = fix_value_host(ssim_fx);
+            float mse = sse / (std::max(1.0f, mask_sum) * 3.0f);
             float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
             if (log_psnr && n_logs < log_cap) log_psnr[n_logs] = psnr;
             if (o.ssim_metric) {
-                const float ssim = (float)ssim_sum / std::max(1.0f, mask_sum);   /* sad_cuda.cu:2443-2448 */
+                const float ssim = ssim_sum / std::max(1.0f, mask_sum);   /* sad_cuda.cu:2443-2448 */
                 if (res && res->log_ssim && n_logs < log_cap) res->log_ssim[n_logs] = ssim;
                 final_ssim = ssim;
                 best_ssim = std::max(best_ssim, ssim);
@@ -442,13 +458,13 @@ static int fit_impl(const SadhFitOptions& o, const float* target_h, const float*
         SAD_TRY(launch_pack_inference(d_sites, n, width, height, d_packed, d_quant, S));
         SAD_TRY(launch_unpack_inference(d_packed, d_quant, n, width, height, d_dec, S));
         SAD_TRY(launch_render(cA0, cA1, d_render, d_dec, iss, n, width, height, cw, ch, S));
-        SAD_TRY(launch_sse64(d_render, d_target, d_mask, d_mse, width, height, S));
-        double sse = 0.0;
-        HIP_TRY(hipMemcpyAsync(&sse, d_mse, sizeof(double), hipMemcpyDeviceToHost, S));
+        SAD_TRY(launch_sse_fix(d_render, d_target, d_mask, d_mse, width, height, S));
+        long long sse_fx[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(sse_fx, d_mse, sizeof(sse_fx), hipMemcpyDeviceToHost, S));
         if (res && res->packed_render)
             HIP_TRY(hipMemcpyAsync(res->packed_render, d_render, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, S));
         HIP_TRY(hipStreamSynchronize(S));
-        const float mse = (float)sse / (std::max(1.0f, mask_sum) * 3.0f);
+        const float mse = fix_value_host(sse_fx) / (std::max(1.0f, mask_sum) * 3.0f);
         packed_psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
     }
     if (res && res->final_render) {

@@ -37,7 +37,7 @@ int launch_site_stats(const uint32_t*, const uint32_t*, const float*, const floa
 int launch_adam(SadSite*, SadAdamState*, float* const[10], float, float, float, float, float, float, float, float,
                 float, uint32_t, uint32_t, int, int, bool, hipStream_t);
 int launch_clear(float* const*, int, uint32_t, hipStream_t);
-int launch_clear64(double* const*, int, uint32_t, hipStream_t);
+int launch_clear_fix(long long*, size_t, hipStream_t);
 int launch_tau_adam(const uint32_t*, const uint32_t*, float*, float*, uint32_t*, float, int, int, int, SadSite*,
                     SadAdamState*, float* const[10], float, float, float, float, float, float, float, float, float,
                     uint32_t, uint32_t, int, int, hipStream_t);
@@ -46,7 +46,7 @@ int launch_tau_diffuse(const uint32_t*, const uint32_t*, const SadSite*, const f
 int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const float* const[8], uint32_t, const float*,
                  int, int, hipStream_t);
 int launch_sse(const float*, const float*, const float*, float*, int, int, hipStream_t);
-int launch_sse64(const float*, const float*, const float*, double*, int, int, hipStream_t);
+int launch_sse_fix(const float*, const float*, const float*, long long*, int, int, hipStream_t);
 int launch_count_active(const SadSite*, uint32_t*, uint32_t, hipStream_t);
 int launch_init_gradient(SadSite*, uint32_t, uint32_t*, const float*, const float*, float, uint32_t, float, float, int,
                          int, hipStream_t);
@@ -116,36 +116,41 @@ int launch_gradients_aos(const uint32_t*, const uint32_t*, const float*, const f
                          float ssim_weight = 0.0f);
 int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
                 hipStream_t stream);
-int launch_ssim64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
-                  hipStream_t stream);
+int launch_ssim_fix(const float* rendered, const float* target, const float* mask, long long* acc, int width,
+                    int height, hipStream_t stream);
 int launch_pack_inference(const SadSite* sites, uint32_t n, int width, int height, SadInferenceSite* out,
                           SadInferenceQuant* quant, hipStream_t stream);
 int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant* quant, uint32_t n, int width,
                             int height, SadSite* out, hipStream_t stream);
-/* the fit's tail (4 tau passes, the last fused with Adam + clear) on double [N][16] gradient lines;
+/* The fit's exact cross-tile sums: gradients as fixed-point lines of FIX_GRAD_STRIDE int64 words per site
+ * ((lo, hi) of values 0-9 in Site field order), removal deltas as one (lo, hi) pair per site, densify
+ * statistics as lines of FIX_STAT_STRIDE words (fix_split, sad_device.h). */
+/* the fit's tail (4 tau passes, the last fused with Adam + clear of the lines);
  * nbr_lists: 8 uint32 per site of scratch for the neighbour lists the passes share */
-int launch_tau_adam_lines64(const uint32_t*, const uint32_t*, double*, float*, float*, float*, float, int, int, int,
-                            SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
-                            uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists, bool lists_built);
-/* the fit's double cross-tile sums: gradient lines and removal deltas, site statistics */
+int launch_tau_adam_fix(const uint32_t*, const uint32_t*, long long*, float*, float*, float*, float, int, int, int,
+                        SadSite*, SadAdamState*, float, float, float, float, float, float, float, float, float,
+                        uint32_t, uint32_t, int, int, hipStream_t, uint32_t* nbr_lists, bool lists_built);
 /* nbr_lists (nullable): the same launch also writes the tau passes' neighbour lists on this field */
-int launch_gradients_lines64(const uint32_t*, const uint32_t*, const float*, const float*, double*, const SadSite*,
-                             float, uint32_t, double*, bool, int, int, int, int, hipStream_t,
-                             const float* rendered = nullptr, float ssim_weight = 0.0f, uint32_t* nbr_lists = nullptr,
-                             float nbr_lambda = 0.0f, int ds = 1);
-int launch_site_stats64(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
-                        double* const[8], int, int, int, int, hipStream_t);
-int launch_split(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const double* const[8], uint32_t, const float*,
-                 int, int, hipStream_t);
-int launch_densify_pairs(const SadSite*, const double*, const double*, SadPair*, uint32_t, float, float, uint32_t,
-                         hipStream_t);
-int launch_prune_pairs(const SadSite*, const double*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
+int launch_gradients_fix(const uint32_t*, const uint32_t*, const float*, const float*, long long*, const SadSite*,
+                         float, uint32_t, long long*, bool, int, int, int, int, hipStream_t,
+                         const float* rendered = nullptr, float ssim_weight = 0.0f, uint32_t* nbr_lists = nullptr,
+                         float nbr_lambda = 0.0f, int ds = 1);
+int launch_site_stats_fix(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float,
+                          uint32_t, long long*, int, int, int, int, hipStream_t);
+int launch_split_fix(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const long long*, uint32_t, const float*,
+                     int, int, hipStream_t);
+int launch_densify_pairs_fix(const SadSite*, const long long*, SadPair*, uint32_t, float, float, uint32_t,
+                             hipStream_t);
+int launch_prune_pairs_fix(const SadSite*, const long long*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
 
 /* batched fit (one launch over every job) */
 int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);
 int launch_vpt_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
 int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, const BatchStep&, hipStream_t);
 int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
+
+/* a fixed-point (lo, hi) sum read back on the host, with the device's rounding (fix_value) */
+inline float fix_value_host(const long long w[2]) { return (float)((double)w[1] + (double)w[0] * 0x1p-32); }
 
 /* capacity and densify/prune percentiles of a fit (sad_cuda.cu:904-1098) */
 int plan(const SadhFitOptions& o, int width, int height, float* d_pct, float* p_pct, int* cap, int* pairs);

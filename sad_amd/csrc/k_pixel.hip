@@ -8,29 +8,35 @@
  * an L2.  Stages:
  *   1. each thread (one pixel) loads its 8 candidate ids and inserts them into
  *      an LDS hash of H slots (multiplicative hash, linear probing, CAS only on
- *      an empty slot).  Ids that find no slot within 8 probes are "overflow";
- *      in the backward modes such a tile then inserts again, one wave at a time,
- *      so that which ids overflow does not depend on wave timing.
+ *      an empty slot).  Ids that find no slot within 8 probes are "pending".
  *   2. one thread per occupied slot loads the 40-byte site once and stores the
  *      per-site terms of the forward (tau, e^a, 1/e^a, r/S, colour) in LDS; the
  *      occupied slots are compacted into a dense list.
  *   3. each thread computes its pixel's softmax over the 8 candidates from the
- *      staged terms.  The render writes the colour here.  The backward kernels
- *      store the weights and the pixel's (pred, dL/dpred) in LDS and take a rank
- *      per (slot, pixel) with an integer LDS atomic on a per-wave counter;
- *      overflow candidates are differentiated right here and go to global
- *      memory with atomics.
- *   4. a scan of the per-slot, per-wave counts turns ranks into positions of a
- *      per-slot entry list (pixel, candidate rank) whose order does not depend
- *      on wave timing, and cuts every slot's list into chunks of CHUNK entries.
+ *      staged terms (pending ids read their site directly).  The render writes
+ *      the colour here.  The backward kernels store the weights and the pixel's
+ *      (pred, dL/dpred) in LDS.
+ *   Then, in the backward modes, rounds of:
+ *   4. a rank per (slot, pixel, candidate) from an integer LDS atomic on a
+ *      per-wave counter, and a scan of the per-slot, per-wave counts, give each
+ *      slot an entry list (pixel, candidate rank) in the order (wave, rank,
+ *      lane); every slot's list is cut into chunks of CHUNK entries.
  *   5. chunk-parallel accumulation: CLANES (4) lanes per chunk differentiate
  *      its (pixel, candidate) pairs and sum in registers; a DPP reduction over
- *      the lane group leaves one partial per chunk in LDS; the partials of each slot are then
- *      summed (one thread per site and parameter) into one global atomic per
- *      (site, parameter) and tile: float for the ABI's float outputs, double
- *      for the fit, whose sums are then independent of the tiles' order.  Chunking balances the work: the entry
- *      counts of a tile's sites range from a few to all 256 pixels, and giving
- *      whole slots to lane groups left most lanes idle behind the largest one.
+ *      the lane group leaves one partial per chunk in LDS; the partials of each
+ *      slot are then summed in chunk order (one thread per site and value) and
+ *      issued as one global atomic per (site, value) and tile: a float add for
+ *      the ABI's float outputs, two int64 adds (fix_split) for the fit.
+ *   6. if some ids were pending, the hash is cleared, the pending ids are
+ *      inserted again and stages 2, 4 and 5 run for them; until none is left.
+ *   Every (tile, site) sum therefore runs over the same entries in the same
+ *   order whatever the hash placed where: the list order follows (wave, rank,
+ *   lane) because the lanes of one LDS atomic instruction are served in lane
+ *   order and a wave's atomics in program order, and the fit's cross-tile sums
+ *   are integer.  A fit repeats bit for bit, and the CPU oracle, which forms
+ *   the same per-tile sums, reproduces it.  Chunking balances the work: the
+ *   entry counts of a tile's sites range from a few to all 256 pixels, and
+ *   giving whole slots to lane groups left most lanes idle behind the largest.
  * Why not LDS float atomics (the reference's tile hash, gradients.cu:422-459):
  * on gfx950, ds_add_f32 measured ~190 cycles per wave instruction per CU
  * against ~5 for ds_add_u32 and ~8 for a read-add-write, and the hashed
@@ -42,24 +48,19 @@
  * matters beyond the render: dL/dlogit = w g.(c - pred) cancels when one
  * candidate dominates a pixel, so ulp-level differences in pred would become
  * large relative differences in the gradient.  The per-pair derivative is
- * sad_grad_terms, the function the oracle evaluates, so GPU and CPU differ only
- * in the order of the fp32 sums.
+ * sad_grad_terms, the function the oracle evaluates (its square roots from the
+ * reproducible sad_rsqrt_nr), so per-pair values agree bit for bit as well.
  */
 #include <type_traits>
 
-/* derivative terms: hardware rsq (~1 ulp) for sqrt(d2) and 1/sqrt(d2); the
- * softmax forward keeps the exact sqrt and exp */
-#ifndef SAD_EXACT_GRAD_MATH
-#define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (isd) = __builtin_amdgcn_rsqf(x); (sq) = (x) * (isd); } while (0)
-#endif
 #include "batch.h"
 
 namespace sadh {
 
 enum PixelMode { MODE_RENDER = 0, MODE_GRAD = 1, MODE_BWD = 2, MODE_STATS = 3 };
 
-constexpr uint16_t SLOT_INVALID = 0xFFFFu;   /* id >= site count or pixel not processed */
-constexpr uint16_t SLOT_OVERFLOW = 0xFFFEu;  /* valid id without a hash slot */
+constexpr uint16_t SLOT_INVALID = 0xFFFFu;   /* id >= site count, pixel not processed, or already summed */
+constexpr uint16_t SLOT_OVERFLOW = 0xFFFEu;  /* valid id without a hash slot this round */
 constexpr uint16_t ENTRY_FIRST = 0x8000u;    /* entry flag: first rank of its site in the pixel */
 
 #ifndef SAD_CHUNK
@@ -72,7 +73,9 @@ constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (6 per lane of 
 constexpr uint32_t CLANES = SAD_CHUNK_LANES;   /* lanes per chunk: 4 or 8 */
 static_assert(CLANES == 4u || CLANES == 8u, "chunks take 4 or 8 lanes");
 
-/* sum over the CLANES-lane group of a lane; every lane of the group gets it */
+/* sum over the CLANES-lane group of a lane; every lane of the group gets it:
+ * (v0 + v1) + (v2 + v3) for 4 lanes (float addition is commutative, so the
+ * lanes' mirrored orders give the same bits) */
 __device__ __forceinline__ float chunk_sum(float v)
 {
     if constexpr (CLANES == 8u) return group8_sum(v);
@@ -103,15 +106,14 @@ struct TileSites {
 template <int H, bool REMOVAL, int NV>
 struct TileLds : TileSites<H> {
     static constexpr uint32_t MAX_CHUNKS = 2048u / CHUNK + (uint32_t)H;
-    uint32_t start[H];         /* first entry of each slot in `list` (first: overflow site ids) */
+    uint32_t start[H];         /* first entry of each slot in `list` */
     /* entries per slot counted per wave, two 16-bit fields per word (waves 2v and 2v+1 in wcount[v]); after
      * the scan, each field holds the offset of that wave's entries within the slot's list */
     uint32_t wcount[2][H];
     uint16_t cstart[H];        /* first chunk of each slot */
     uint32_t n_chunks;
     uint32_t stot[4][2];       /* stage-4 entry and chunk totals of each scanning wave */
-    uint32_t n_ovf;            /* pairs whose id found no hash slot */
-    uint32_t hstate;           /* nonzero: an id found no slot in its 8 probes */
+    uint32_t hstate[2];        /* [round & 1] nonzero: some id is pending after this round's inserts */
     /* per pixel: slot of each candidate rank (SLOT8_NONE if unused); read only to sum a site's weight over
      * its ranks in a pixel (removal deltas, statistics) */
     uint8_t slot[(REMOVAL || NV == 8) ? 256 : 1][8];
@@ -119,9 +121,9 @@ struct TileLds : TileSites<H> {
     float4 px[256];            /* per pixel: pred.rgb, err (STATS) */
     float4 pg[256];            /* per pixel: dL/dpred.rgb */
     float4 pt[REMOVAL ? 256 : 1];  /* per pixel: target.rgb (removal deltas) */
-    uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST); first: overflow pairs */
+    uint16_t list[256 * 8];    /* per-slot entry lists: pixel * 8 + rank (| ENTRY_FIRST) */
     uint8_t chunk_slot[MAX_CHUNKS];
-    float part[MAX_CHUNKS][NV];    /* per-chunk partial sums; first: overflow pair values */
+    float part[MAX_CHUNKS][NV];    /* per-chunk partial sums */
 };
 
 struct SiteTerms {
@@ -200,11 +202,11 @@ struct PixelArgs {
     uint32_t list_blocks;
     float* g[10];              /* DST_SOA: value j of site i accumulates at g[j] + i * gstride */
     float* gline;              /* DST_LINES: value j of site i accumulates at gline + i * gstride + j */
-    double* gline64;           /* DST_LINES64: value j of site i at gline64 + i * 16 + j */
-    double* g64[8];            /* DST_SOA64: value j of site i at g64[j] + i */
-    double* removal64;         /* DST_LINES64 with removal: removal delta of site i at removal64[i] */
+    long long* gfix;           /* DST_FIX: (lo, hi) of gradient j of site i at gfix + i * FIX_GRAD_STRIDE + 2 j */
+    long long* rfix;           /* DST_FIX with removal: (lo, hi) of the removal delta of site i at rfix + 2 i */
+    long long* sfix;           /* DST_FIXSTATS: (lo, hi) of statistic j of site i at sfix + i * FIX_STAT_STRIDE + 2 j */
     uint32_t gstride;          /* 1: SoA arrays; 10: AoS [N][10] (BWD); 16: AoS [N][16] lines */
-    float* removal;            /* GRAD with removal */
+    float* removal;            /* GRAD with removal, float destinations */
     float iss, inv_scale;
     uint32_t n;
     int width, height, cw, ch;
@@ -215,30 +217,63 @@ template <int MODE, bool REMOVAL, int H>
 using PixelLds = typename std::conditional<MODE == MODE_RENDER, TileSites<H>,
                                            TileLds<H, REMOVAL, NumValues<MODE, REMOVAL>::value>>::type;
 
-/* Where a tile's per-site sums go.  The fit accumulates across tiles in
- * double (DST_LINES64 for gradients and removal deltas, DST_SOA64 for the
- * densify statistics) and its consumers round once to float, as the oracle
- * rounds its double sums: the order of the cross-tile atomics then changes a
- * result only when a double sum lies within 1e-16 of a float rounding
- * boundary, so fits repeat run to run.  The float kinds serve the C ABI's
- * float outputs.  AoS lines index by arithmetic, so a PixelArgs built in
- * registers is never indexed by a lane-varying value. */
-enum PixelDst { DST_SOA = 0, DST_LINES = 1, DST_LINES64 = 2, DST_SOA64 = 3 };
+/* Where a tile's per-site sums go.  The float kinds serve the C ABI's float
+ * outputs (the reference's layouts; the order of the float atomics rounds
+ * differently run to run, as the reference's do).  The fit accumulates its
+ * gradients, removal deltas (DST_FIX) and densify statistics (DST_FIXSTATS)
+ * in the exact two-word fixed point of fix_split, so its sums do not depend
+ * on the tiles' order.  AoS lines index by arithmetic, so a PixelArgs built
+ * in registers is never indexed by a lane-varying value. */
+enum PixelDst { DST_SOA = 0, DST_LINES = 1, DST_FIX = 2, DST_FIXSTATS = 3 };
 
 template <int DST>
 __device__ __forceinline__ void acc_add(const PixelArgs& A, uint32_t j, uint32_t id, float v)
 {
-    if constexpr (DST == DST_LINES64) unsafeAtomicAdd(A.gline64 + (size_t)id * 16u + j, (double)v);
-    else if constexpr (DST == DST_SOA64) unsafeAtomicAdd(A.g64[j] + id, (double)v);
-    else if constexpr (DST == DST_LINES) unsafeAtomicAdd(A.gline + (size_t)id * A.gstride + j, v);
+    if constexpr (DST == DST_LINES) unsafeAtomicAdd(A.gline + (size_t)id * A.gstride + j, v);
     else unsafeAtomicAdd(A.g[j] + (size_t)id * A.gstride, v);
 }
 
-template <int DST>
-__device__ __forceinline__ void removal_add(const PixelArgs& A, uint32_t id, float v)
+/* Round r's inserts of the ids still pending (slot SLOT_OVERFLOW); summed ids
+ * leave the table (SLOT_INVALID).  Returns whether some id is still pending. */
+template <class LDS, int H>
+__device__ __forceinline__ bool insert_pending(LDS& L, const uint32_t ids[8], uint16_t slots[8])
 {
-    if constexpr (DST == DST_LINES64) unsafeAtomicAdd(A.removal64 + id, (double)v);
-    else unsafeAtomicAdd(A.removal + id, v);
+    bool ovf = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (slots[k] == SLOT_OVERFLOW) {
+            slots[k] = hash_insert<LDS, H>(L, ids[k]);
+            ovf = ovf || slots[k] == SLOT_OVERFLOW;
+        } else {
+            slots[k] = SLOT_INVALID;
+        }
+    }
+    return ovf;
+}
+
+/* stage 2: per-site terms of the occupied slots; dense list of them (backward modes) */
+template <int MODE, int H, class LDS>
+__device__ __forceinline__ void stage_sites(const PixelArgs& A, LDS& L, uint32_t tid)
+{
+    for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
+        uint32_t id = L.key[s];
+        if (id == SAD_EMPTY) continue;
+        SiteTerms t = site_terms(A.sites, id, A.inv_scale);
+        L.p0[s] = t.p0;
+        L.p1[s] = t.p1;
+        L.p2[s] = t.p2;
+    }
+    if (MODE != MODE_RENDER && tid < 64u) {
+        uint32_t base = 0u;
+        const unsigned long long lt = (tid == 0u) ? 0ull : (~0ull >> (64u - tid));
+        for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
+            bool occ = L.key[s0 + tid] != SAD_EMPTY;
+            unsigned long long b = __ballot(occ);
+            if (occ) L.occ[base + (uint32_t)__popcll(b & lt)] = (uint16_t)(s0 + tid);
+            base += (uint32_t)__popcll(b);
+        }
+        if (tid == 0u) L.n_occ = base;
+    }
 }
 
 /* One 16x16 tile (index `tile` of A's image) of any mode; 256 threads, one pixel each. */
@@ -247,6 +282,9 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
 {
     constexpr int NV = NumValues<MODE, REMOVAL>::value;
     static_assert(H <= 128, "per-pixel slot table holds 7-bit slots");
+    static_assert(MODE != MODE_STATS || DST == DST_FIXSTATS || DST == DST_SOA,
+                  "statistics go to DST_FIXSTATS or float arrays");
+    static_assert(DST != DST_FIXSTATS || MODE == MODE_STATS, "DST_FIXSTATS holds statistics");
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = tid >> 6;   /* wave of the block */
     const int tx0 = (int)(tile % A.tiles_x) * 16, ty0 = (int)(tile / A.tiles_x) * 16;
@@ -266,8 +304,8 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     }
     if constexpr (MODE != MODE_RENDER) {
         if (tid == 0u) {
-            L.n_ovf = 0u;
-            L.hstate = 0u;
+            L.hstate[0] = 0u;
+            L.hstate[1] = 0u;
         }
     }
     __syncthreads();
@@ -302,55 +340,9 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         for (int k = 0; k < 8; ++k)
             slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
     }
-    if constexpr (MODE != MODE_RENDER) {
-        bool ovf = false;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) ovf = ovf || slots[k] == SLOT_OVERFLOW;
-        if (ovf) L.hstate = 1u;
-    }
     __syncthreads();
 
-    /* In the backward modes a site's sum over slot entries rounds differently
-     * from its sum over direct pairs, so which ids get a slot must not depend
-     * on the order in which the waves insert.  When some id missed its probes,
-     * the tile inserts again one wave at a time: within a wave the inserts of
-     * rank k precede those of rank k + 1, and the lanes of one LDS atomic are
-     * served in a fixed order. */
-    if constexpr (MODE != MODE_RENDER) {
-        if (L.hstate) {
-            for (uint32_t s = tid; s < (uint32_t)H; s += 256u) L.key[s] = SAD_EMPTY;
-            __syncthreads();
-            for (uint32_t turn = 0; turn < 4u; ++turn) {
-                if (wv == turn) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k)
-                        slots[k] = (act && ids[k] < A.n) ? hash_insert<decltype(L), H>(L, ids[k]) : SLOT_INVALID;
-                }
-                __syncthreads();
-            }
-        }
-    }
-
-    /* stage 2: per-site terms of the occupied slots; dense list of them */
-    for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
-        uint32_t id = L.key[s];
-        if (id == SAD_EMPTY) continue;
-        SiteTerms t = site_terms(A.sites, id, A.inv_scale);
-        L.p0[s] = t.p0;
-        L.p1[s] = t.p1;
-        L.p2[s] = t.p2;
-    }
-    if (MODE != MODE_RENDER && tid < 64u) {
-        uint32_t base = 0u;
-        const unsigned long long lt = (tid == 0u) ? 0ull : (~0ull >> (64u - tid));
-        for (uint32_t s0 = 0; s0 < (uint32_t)H; s0 += 64u) {
-            bool occ = L.key[s0 + tid] != SAD_EMPTY;
-            unsigned long long b = __ballot(occ);
-            if (occ) L.occ[base + (uint32_t)__popcll(b & lt)] = (uint16_t)(s0 + tid);
-            base += (uint32_t)__popcll(b);
-        }
-        if (tid == 0u) L.n_occ = base;
-    }
+    stage_sites<MODE, H>(A, L, tid);
     __syncthreads();
 
     /* stage 3: softmax of this pixel (render.cu:34-91) */
@@ -441,99 +433,35 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     L.px[tid] = make_float4(pr, pgr, pb, err);
     L.pg[tid] = make_float4(gr, gg, gb, 0.0f);
     if (REMOVAL) L.pt[tid] = make_float4(tr, tg, tb, 0.0f);
-    /* An entry's place in its slot's list is (entries of earlier waves) + (its rank among this wave's).  The
-     * per-wave counters make the list order, and with it every float sum over the list, independent of the
-     * order in which the tile's waves reach this point: within one wave the atomics run in program order,
-     * and the lanes of one LDS atomic instruction are served in a fixed order. */
+    /* the pairs that are summed keep their slot; a valid id without a slot waits for a later round */
+    bool pending = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        L.w[tid][k] = w[k];
+        if (!(contrib && valid[k])) slots[k] = SLOT_INVALID;
+        pending = pending || slots[k] == SLOT_OVERFLOW;
+    }
+    if (pending) L.hstate[0] = 1u;
+
     const uint32_t wshift = 16u * (wv & 1u);
+    for (uint32_t round = 0;; ++round) {
+    /* stage 4: an entry's place in its slot's list is (entries of earlier waves) + (its rank among this
+     * wave's).  The per-wave counters make the list order, and with it every float sum over the list,
+     * independent of the order in which the tile's waves reach this point: within one wave the atomics run
+     * in program order, and the lanes of one LDS atomic instruction are served in lane order. */
     uint32_t rank[8];
     uint8_t sl[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        L.w[tid][k] = w[k];
         const uint16_t s = slots[k];
-        const bool use = contrib && valid[k];
-        sl[k] = (use && s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
+        sl[k] = (s < (uint16_t)H) ? (uint8_t)s : SLOT8_NONE;
         rank[k] = (sl[k] != SLOT8_NONE) ? ((atomicAdd(&L.wcount[wv >> 1][s], 1u << wshift) >> wshift) & 0xFFFFu)
                                         : 0u;
         if constexpr (MODE == MODE_STATS || REMOVAL) L.slot[tid][k] = sl[k];
-        if (use && s == SLOT_OVERFLOW) L.list[atomicAdd(&L.n_ovf, 1u)] = (uint16_t)((tid << 3) | (uint32_t)k);
     }
     __syncthreads();
 
-    /* stage 3c: pairs whose id found no hash slot (many only while the candidate
-     * field is still random).  Per round of R pairs, one lane differentiates a
-     * pair into LDS, then the values are issued with consecutive lanes on
-     * consecutive values of one site, so a site's values in an AoS layout are one
-     * memory request (atomics to one 64-byte line in one instruction coalesce). */
-    {
-        constexpr uint32_t R = (uint32_t)H < 128u ? (uint32_t)H : 128u;
-        const uint32_t n_ovf = L.n_ovf;
-        float* stage = &L.part[0][0];
-        for (uint32_t base = 0; base < n_ovf; base += R) {
-            const uint32_t cnt = min(R, n_ovf - base);
-            if (tid < cnt) {
-                const uint32_t ent = L.list[base + tid];
-                const uint32_t p = ent >> 3, k = ent & 7u;
-                const int ex = tx0 + (int)(p & 15u), ey = ty0 + (int)(p >> 4);
-                const int cell = sad_cand_index(ex, ey, A.width, A.height, A.cw, A.ch);
-                const uint4 ca = reinterpret_cast<const uint4*>(A.cand0)[cell];
-                const uint4 cb = reinterpret_cast<const uint4*>(A.cand1)[cell];
-                const uint32_t pid[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-                const uint32_t id = pid[k];
-                /* a repeated id shares validity with this one; invalid weights are 0 */
-                bool first = true;
-                float wt = 0.0f;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (pid[j] == id) {
-                        wt += L.w[p][j];
-                        if ((uint32_t)j < k) first = false;
-                    }
-                }
-                const float4 pp = L.px[p];
-                const float4 gq = L.pg[p];
-                const float fx = (float)ex, fy = (float)ey;
-                float v[NV];
-#pragma unroll
-                for (int j = 0; j < NV; ++j) v[j] = 0.0f;
-                if constexpr (MODE == MODE_STATS) {
-                    if (first && wt > 0.0f) {
-                        const float er = pp.w;
-                        const float we = wt * fminf(er, 1.0f);
-                        v[0] = wt; v[1] = wt * er; v[2] = we; v[3] = we * fx; v[4] = we * fy;
-                        v[5] = we * fx * fx; v[6] = we * fx * fy; v[7] = we * fy * fy;
-                    }
-                } else {
-                    const SiteTerms t = site_terms(A.sites, id, A.inv_scale);
-                    float c[10];
-                    candidate_grad(t, fx, fy, L.w[p][k], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss, A.inv_scale, c);
-#pragma unroll
-                    for (int j = 0; j < 10; ++j) v[j] = c[j];
-                    if (MODE == MODE_GRAD && REMOVAL && first && wt > 0.0f) {
-                        const float4 tq = L.pt[p];
-                        v[NV - 1] = sad_removal_delta(pp.x, pp.y, pp.z, tq.x, tq.y, tq.z, t.p2.x, t.p2.y, t.p2.z, wt);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < NV; ++j) stage[tid * (uint32_t)NV + (uint32_t)j] = v[j];
-                L.start[tid] = id;
-            }
-            __syncthreads();
-            for (uint32_t q = tid; q < cnt * (uint32_t)NV; q += 256u) {
-                const uint32_t e = q / (uint32_t)NV, j = q - e * (uint32_t)NV;
-                const float v = stage[q];
-                if (v != 0.0f) {
-                    const uint32_t id = L.start[e];
-                    if (MODE == MODE_GRAD && REMOVAL && j == 10u) removal_add<DST>(A, id, v);
-                    else acc_add<DST>(A, j, id, v);
-                }
-            }
-            __syncthreads();
-        }
-    }
-
-    /* stage 4 (waves 0..H/64-1, 64 slots each): entry offsets and chunk offsets =
+    /* (waves 0..H/64-1, 64 slots each): entry offsets and chunk offsets =
      * exclusive scans of the per-slot entry and chunk counts; chunk -> slot table.
      * Each wave scans its slots, posts its totals, and after a barrier adds the
      * totals of the waves before it. */
@@ -660,25 +588,65 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     }
     __syncthreads();
 
-    /* per (site, value): sum the slot's chunk partials, one global atomic */
+    /* per (site, value): sum the slot's chunk partials in chunk order, then the global atomics */
     const uint32_t n_occ = L.n_occ;
-    for (uint32_t i = tid; i < n_occ * (uint32_t)NV; i += 256u) {
-        const uint32_t u = i / (uint32_t)NV, j = i - u * (uint32_t)NV;
-        const uint32_t s = L.occ[u];
-        const uint32_t c = L.count[s];
-        if (c == 0u || !(L.p0[s].x >= 0.0f)) continue;
-        const uint32_t q0 = L.cstart[s], q1 = q0 + (c + CHUNK - 1u) / CHUNK;
-        float v = 0.0f;
-        for (uint32_t q = q0; q < q1; ++q) v += L.part[q][j];
-        const uint32_t id = L.key[s];
-        if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
-            if (v != 0.0f) removal_add<DST>(A, id, v);
-        } else {
-            acc_add<DST>(A, j, id, v);
+    if constexpr (DST == DST_FIX || DST == DST_FIXSTATS) {
+        /* one lane per (site, value, word): a site's (lo, hi) pairs are consecutive lanes and addresses */
+        for (uint32_t i = tid; i < n_occ * (uint32_t)(2 * NV); i += 256u) {
+            const uint32_t u = i / (uint32_t)(2 * NV), r = i - u * (uint32_t)(2 * NV);
+            const uint32_t j = r >> 1, hiw = r & 1u;
+            const uint32_t s = L.occ[u];
+            const uint32_t c = L.count[s];
+            if (c == 0u || !(L.p0[s].x >= 0.0f)) continue;
+            const uint32_t qa = L.cstart[s], qb = qa + (c + CHUNK - 1u) / CHUNK;
+            float v = 0.0f;
+            for (uint32_t q = qa; q < qb; ++q) v += L.part[q][j];
+            long long lo, hi;
+            fix_split(v, &lo, &hi);
+            const size_t id = L.key[s];
+            long long* dst;
+            if (DST == DST_FIXSTATS) dst = A.sfix + id * FIX_STAT_STRIDE + 2u * j;
+            else if (REMOVAL && j == 10u) dst = A.rfix + 2u * id;
+            else dst = A.gfix + id * FIX_GRAD_STRIDE + 2u * j;
+            fix_atomic_add(dst + hiw, hiw ? hi : lo);
+        }
+    } else {
+        for (uint32_t i = tid; i < n_occ * (uint32_t)NV; i += 256u) {
+            const uint32_t u = i / (uint32_t)NV, j = i - u * (uint32_t)NV;
+            const uint32_t s = L.occ[u];
+            const uint32_t c = L.count[s];
+            if (c == 0u || !(L.p0[s].x >= 0.0f)) continue;
+            const uint32_t qa = L.cstart[s], qb = qa + (c + CHUNK - 1u) / CHUNK;
+            float v = 0.0f;
+            for (uint32_t q = qa; q < qb; ++q) v += L.part[q][j];
+            const uint32_t id = L.key[s];
+            if (MODE == MODE_GRAD && REMOVAL && j == 10u) {
+                if (v != 0.0f) unsafeAtomicAdd(A.removal + id, v);
+            } else {
+                acc_add<DST>(A, j, id, v);
+            }
         }
     }
+
+    /* stage 6: the next round, for the ids that found no slot */
+    const uint32_t more = L.hstate[round & 1u];
+    if (!more) break;
+    __syncthreads();
+    for (uint32_t s = tid; s < (uint32_t)H; s += 256u) {
+        L.key[s] = SAD_EMPTY;
+        L.wcount[0][s] = 0u;
+        L.wcount[1][s] = 0u;
+    }
+    if (tid == 0u) L.hstate[(round + 1u) & 1u] = 0u;
+    __syncthreads();
+    if (insert_pending<decltype(L), H>(L, ids, slots)) L.hstate[(round + 1u) & 1u] = 1u;
+    __syncthreads();
+    stage_sites<MODE, H>(A, L, tid);
+    __syncthreads();
+    }  /* rounds */
     }  /* backward modes */
 }
+
 
 template <int MODE, bool REMOVAL, int H, int DST>
 __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
@@ -711,8 +679,8 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.sites = J.sites;
     A.target = J.target;
     A.mask = J.mask;
-    A.gline64 = J.gacc;
-    A.removal64 = J.removal;
+    A.gfix = J.gacc;
+    A.rfix = J.removal;
     A.rendered = J.rendered;
     A.ssim_weight = J.ssim_weight;
     A.gstride = 16u;
@@ -725,7 +693,7 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     A.ch = J.ch;
     A.tiles_x = J.tiles_x;
     A.ntiles = J.ntiles;
-    pixel_tile<MODE_GRAD, REMOVAL, H, DST_LINES64>(A, f - J.tile_begin, L);
+    pixel_tile<MODE_GRAD, REMOVAL, H, DST_FIX>(A, f - J.tile_begin, L);
 }
 
 /* ------------------------------------------------------------ launchers -- */
@@ -844,19 +812,17 @@ int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const flo
     return launch_pixel<MODE_GRAD, false, DST_LINES>(A, stream, "compute_gradients_tiled(aos)");
 }
 
-/* the fit's gradients: double AoS [N][16] lines and double removal deltas */
-int launch_gradients_lines64(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
-                             double* gacc, const SadSite* sites, float iss, uint32_t n, double* removal,
-                             bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
-                             const float* rendered, float ssim_weight, uint32_t* nbr_lists, float nbr_lambda,
-                             int ds)
+/* the fit's gradients: fixed-point (lo, hi) lines and removal pairs */
+int launch_gradients_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                         long long* gacc, const SadSite* sites, float iss, uint32_t n, long long* removal,
+                         bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
+                         const float* rendered, float ssim_weight, uint32_t* nbr_lists, float nbr_lambda, int ds)
 {
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
     A.rendered = rendered; A.ssim_weight = ssim_weight;
-    A.gline64 = gacc;
-    A.removal64 = removal;
-    A.gstride = 16u;
+    A.gfix = gacc;
+    A.rfix = removal;
     A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     if (nbr_lists && n > 0u) {
@@ -866,8 +832,8 @@ int launch_gradients_lines64(const uint32_t* cand0, const uint32_t* cand1, const
         A.nbr_lambda = nbr_lambda;
         A.nbr_inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
     }
-    if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_LINES64>(A, stream, "compute_gradients(lines64)");
-    return launch_pixel<MODE_GRAD, false, DST_LINES64>(A, stream, "compute_gradients(lines64)");
+    if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_FIX>(A, stream, "compute_gradients(fix)");
+    return launch_pixel<MODE_GRAD, false, DST_FIX>(A, stream, "compute_gradients(fix)");
 }
 
 int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
@@ -895,17 +861,16 @@ int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float*
     return launch_pixel<MODE_STATS, false>(A, stream, "compute_site_stats");
 }
 
-/* the fit's densify statistics, accumulated in double */
-int launch_site_stats64(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
-                        const SadSite* sites, float iss, uint32_t n, double* const st[8], int width, int height,
-                        int cw, int ch, hipStream_t stream)
+/* the fit's densify statistics, in fixed-point (lo, hi) lines of FIX_STAT_STRIDE words */
+int launch_site_stats_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
+                          const SadSite* sites, float iss, uint32_t n, long long* stats, int width, int height,
+                          int cw, int ch, hipStream_t stream)
 {
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
-    for (int j = 0; j < 8; ++j) A.g64[j] = st[j];
-    A.gstride = 1u;
+    A.sfix = stats;
     A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    return launch_pixel<MODE_STATS, false, DST_SOA64>(A, stream, "compute_site_stats(double)");
+    return launch_pixel<MODE_STATS, false, DST_FIXSTATS>(A, stream, "compute_site_stats(fix)");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream);

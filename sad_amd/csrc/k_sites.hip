@@ -7,6 +7,7 @@
  * and written whole with 8- and 16-byte accesses); gradients are SoA.
  */
 #include <algorithm>
+#include <type_traits>
 
 #include "batch.h"
 
@@ -14,14 +15,14 @@ namespace sadh {
 
 /* ---------------------------------------------------------------- Adam --
  * adamUpdateKernel, adam.cu:5-180.  b1c = 1/(1-beta1^t), b2c = 1/(1-beta2^t)
- * arrive precomputed (host powf, once per launch).  CLEAR zeroes the ten
+ * arrive precomputed (host sad_powi_f, once per launch).  CLEAR zeroes the ten
  * gradient entries after reading them (adam_encoder.metal:117-127). */
 struct AdamArgs {
     SadSite* sites;
     SadAdamState* st;
     float* g[10];              /* gradient j of site i at g[j][i * gstride] */
     uint32_t gstride;          /* 1: SoA arrays; 16: AoS [N][16] lines */
-    double* g64;               /* non-null: gradient j of site i at g64[i * 16 + j] instead (double lines) */
+    long long* gfix;           /* non-null: gradient j of site i is the (lo, hi) pair at gfix + i * FIX_GRAD_STRIDE + 2 j */
     float lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso;
     float beta1, beta2, eps, b1c, b2c, norm, maxx, maxy;
     uint32_t n;
@@ -91,24 +92,20 @@ __device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const f
 }
 
 /* site i's ten gradients times norm, with log_tau replaced by `tau`, then
- * zeroed at the source.  Double lines round once to float here: the
- * cross-tile sums that filled them are exact to well below float precision,
- * so the float seen by Adam does not depend on the order of the atomics. */
+ * zeroed at the source.  Fixed-point lines round once to float here: the
+ * cross-tile sums that filled them are integer, so the float seen by Adam does
+ * not depend on the order of the atomics. */
 __device__ __forceinline__ void take_gradients(const AdamArgs& A, uint32_t i, float tau, float g[10])
 {
-    if (A.g64) {
-        double2* line = reinterpret_cast<double2*>(A.g64 + (size_t)i * 16u);
-        double v[10];
+    if (A.gfix) {
+        longlong2* line = reinterpret_cast<longlong2*>(A.gfix + (size_t)i * FIX_GRAD_STRIDE);
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const double2 d = line[j];
-            v[2 * j] = d.x;
-            v[2 * j + 1] = d.y;
+        for (int j = 0; j < 10; ++j) {
+            const longlong2 d = line[j];
+            g[j] = ((j == 2) ? tau : fix_value(d.x, d.y)) * A.norm;
         }
 #pragma unroll
-        for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? tau : (float)v[j]) * A.norm;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) line[j] = make_double2(0.0, 0.0);
+        for (int j = 0; j < 10; ++j) line[j] = make_longlong2(0, 0);
         return;
     }
 #pragma unroll
@@ -195,8 +192,8 @@ __device__ __forceinline__ void tau_site(uint32_t i, const uint4* __restrict__ c
 
 template <typename TR, typename TI>
 __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                              const SadSite* __restrict__ sites, const TR* __restrict__ graw,
-                                              const TI* __restrict__ gin, float* __restrict__ gout,
+                                              const SadSite* __restrict__ sites, TR graw, TI gin,
+                                              float* __restrict__ gout,
                                               float* __restrict__ raw_copy, uint32_t n, float lambda, int width,
                                               int height, float inv_ds, uint32_t in_stride, uint4* __restrict__ nbr)
 {
@@ -234,10 +231,11 @@ __device__ __forceinline__ void tau_site_list(uint32_t i, const uint4* __restric
     gout[i] = (raw + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
-/* one pass from the list: the value tau_site computes (graw and gin hold site i at [i * stride]) */
+/* one pass from the list: the value tau_site computes (graw and gin hold site i at [i * stride]; float
+ * arrays or FixCol columns of fixed-point lines) */
 template <typename TR, typename TI>
-__device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, const TR* __restrict__ graw,
-                                               const TI* __restrict__ gin, float lambda, uint32_t stride = 1u)
+__device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restrict__ nbr, TR graw, TI gin,
+                                               float lambda, uint32_t stride = 1u)
 {
     const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
     if (a.x == NBR_COPY) return (float)gin[i * stride];
@@ -256,7 +254,7 @@ __device__ __forceinline__ float tau_from_list(uint32_t i, const uint4* __restri
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tau_list_first(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
-                                                        const SadSite* __restrict__ sites, const T* __restrict__ g2,
+                                                        const SadSite* __restrict__ sites, T g2,
                                                         float* __restrict__ gout, float* __restrict__ raw_copy,
                                                         uint32_t n, float lambda, int width, int height, float inv_ds,
                                                         uint4* __restrict__ nbr, uint32_t in_stride)
@@ -292,8 +290,7 @@ __global__ __launch_bounds__(256) void k_tau_list_last_adam(const uint4* __restr
  * neighbour k's value; the group's first lane sums them in slot order. */
 template <typename TR, typename TI>
 __device__ __forceinline__ float tau_two_passes8(uint32_t i, uint32_t lane, const uint4* __restrict__ nbr,
-                                                 const TR* __restrict__ graw, const TI* __restrict__ gin,
-                                                 float lambda, uint32_t stride)
+                                                 TR graw, TI gin, float lambda, uint32_t stride)
 {
     const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
     const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -324,16 +321,16 @@ __device__ __forceinline__ float tau_two_passes8(uint32_t i, uint32_t lane, cons
  * log_tau column of the gradient lines, saves raw, writes the pass-2 value.
  * 8 threads per site; blockDim is a multiple of 64. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_tau_list_12_built(const uint4* __restrict__ nbr, const T* __restrict__ g2,
+__global__ __launch_bounds__(256) void k_tau_list_12_built(const uint4* __restrict__ nbr, T g2,
                                                            float* __restrict__ gout, float* __restrict__ raw_copy,
                                                            uint32_t n, float lambda)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = t >> 3, lane = t & 7u;
     if (i >= n) return;                        /* whole groups of 8 leave together */
-    const float v = tau_two_passes8(i, lane, nbr, g2, g2, lambda, 16u);
+    const float v = tau_two_passes8(i, lane, nbr, g2, g2, lambda, 1u);
     if (lane == 0u) {
-        raw_copy[i] = (float)g2[i * 16u];
+        raw_copy[i] = (float)g2[i];
         gout[i] = v;
     }
 }
@@ -382,21 +379,22 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
     const float inv_ds = 1.0f / fmaxf((float)J.ds, 1.0f);
     uint4* nbr = reinterpret_cast<uint4*>(J.tau_nbr);
     if (pass == 0u) {
-        tau_site_list(i, c0, c1, J.sites, J.gacc + 2, J.gacc + 2, J.tau_tmp, J.tau_raw, J.n, js.lambda, J.cw, J.ch,
-                      inv_ds, 16u, nbr);
+        const FixCol g2{J.gacc + 4, FIX_GRAD_STRIDE};
+        tau_site_list(i, c0, c1, J.sites, g2, g2, J.tau_tmp, J.tau_raw, J.n, js.lambda, J.cw, J.ch, inv_ds, 1u,
+                      nbr);
     } else if (pass == 1u) {
-        J.tau_tmp2[i] = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda);
+        J.tau_tmp2[i] = tau_from_list(i, nbr, (const float*)J.tau_raw, (const float*)J.tau_tmp, js.lambda);
     } else if (pass == 2u) {
-        J.tau_tmp[i] = tau_from_list(i, nbr, J.tau_raw, J.tau_tmp2, js.lambda);
+        J.tau_tmp[i] = tau_from_list(i, nbr, (const float*)J.tau_raw, (const float*)J.tau_tmp2, js.lambda);
     } else {
         AdamArgs A;
         A.sites = J.sites; A.st = J.adam;
-        A.g64 = J.gacc;
+        A.gfix = J.gacc;
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
         A.b1c = js.b1c; A.b2c = js.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
         float g[10];
-        take_gradients(A, i, tau_from_list(i, nbr, J.tau_raw, J.tau_tmp, js.lambda), g);
+        take_gradients(A, i, tau_from_list(i, nbr, (const float*)J.tau_raw, (const float*)J.tau_tmp, js.lambda), g);
         adam_site(A, i, g);
     }
 }
@@ -423,8 +421,8 @@ __global__ void k_update_jobs(BatchJob* jobs, JobUpdate u)
 /* computeDensifyScorePairsKernel, densify_prune.cu:223-254.  pow(m, alpha) is
  * the exact exp(alpha * log m) of sad_math.h so keys match the CPU bit for bit. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict__ sites, const T* __restrict__ mass,
-                                                       const T* __restrict__ energy, SadPair* __restrict__ pairs,
+__global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict__ sites, T mass, T energy,
+                                                       SadPair* __restrict__ pairs,
                                                        uint32_t n, float min_mass, float alpha, uint32_t count)
 {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -443,7 +441,7 @@ __global__ __launch_bounds__(256) void k_densify_pairs(const SadSite* __restrict
 
 /* computePruneScorePairsKernel, densify_prune.cu:257-281 */
 template <typename T>
-__global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__ sites, const T* __restrict__ rdelta,
+__global__ __launch_bounds__(256) void k_prune_pairs(const SadSite* __restrict__ sites, T rdelta,
                                                      SadPair* __restrict__ pairs, uint32_t n, float delta_norm,
                                                      uint32_t count)
 {
@@ -482,7 +480,7 @@ struct SplitArgs {
     SadSite* sites;
     SadAdamState* st;
     const uint32_t* idx;
-    const T *mass, *ew, *ewx, *ewy, *ewxx, *ewxy, *ewyy;
+    T mass, ew, ewx, ewy, ewxx, ewxy, ewyy;        /* float arrays or FixCol columns */
     const float* target;
     uint32_t k, cur;
     int width, height;
@@ -509,14 +507,14 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs<T> A)
         float exx = fmaxf((float)A.ewxx[pi] / w - mx * mx, 1e-4f);
         float exy = (float)A.ewxy[pi] / w - mx * my;
         float eyy = fmaxf((float)A.ewyy[pi] / w - my * my, 1e-4f);
-        float th = 0.5f * atan2f(2.0f * exy, exx - eyy);
-        ax = cosf(th);
-        ay = sinf(th);
+        /* the library's own atan2 and sincos (sad_math.h), so children are bit-identical to the oracle's */
+        float th = 0.5f * sad_atan2f(2.0f * exy, exx - eyy);
+        sad_sincosf(th, &ay, &ax);
         float tr = exx + eyy;
         float disc = sqrtf(fmaxf(0.0f, 0.25f * (exx - eyy) * (exx - eyy) + exy * exy));
         float l1 = fmaxf(1e-4f, 0.5f * tr + disc);
         float l2 = fmaxf(1e-4f, 0.5f * tr - disc);
-        la = sad_clampf(-0.5f * logf(l1 / l2), -2.0f, 2.0f);
+        la = sad_clampf(-0.5f * sad_logf(l1 / l2), -2.0f, 2.0f);
     } else {
         int ix = sad_clampi((int)parent.pos_x, 0, W - 1);
         int iy = sad_clampi((int)parent.pos_y, 0, Hh - 1);
@@ -597,15 +595,15 @@ __global__ __launch_bounds__(256) void k_hilbert_order(const SadPair* __restrict
 /* ------------------------------------------------------------- metrics --- */
 
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
-__global__ void k_zero_dword(unsigned long long* p) { *p = 0ull; }
 
-/* computePSNRKernel, utils.cu:5-26: block sum of squared error, one atomic per block */
-/* T = double: the block partials meet in a double sum, which does not depend on the blocks' order */
-template <typename T>
-__global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const float* __restrict__ t,
-                                             const float* __restrict__ mask, T* __restrict__ acc, uint32_t np)
+/* computePSNRKernel, utils.cu:5-26: block sum of squared error, one atomic per block.
+ * Each block's partial is a fixed tree (a shuffle-down tree per wave, then
+ * (w0 + w1) + (w2 + w3)); T = float is the reference's float atomic, and the
+ * fixed-point kind (k_sse_fix) adds the partial as an exact (lo, hi) pair, so
+ * the fit's PSNR log does not depend on the blocks' order. */
+__device__ __forceinline__ float sse_block(const float* __restrict__ r, const float* __restrict__ t,
+                                           const float* __restrict__ mask, uint32_t np, float* part)
 {
-    __shared__ float part[4];
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     float v = 0.0f;
     if (i < np && mask[i] > 0.0f) {
@@ -615,7 +613,34 @@ __global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const 
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)((part[0] + part[1]) + (part[2] + part[3])));
+    return (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sse_fix(const float* __restrict__ r, const float* __restrict__ t,
+                                                 const float* __restrict__ mask, long long* __restrict__ acc,
+                                                 uint32_t np)
+{
+    __shared__ float part[4];
+    const float v = sse_block(r, t, mask, np, part);
+    if (threadIdx.x < 2u) {
+        long long lo, hi;
+        fix_split(v, &lo, &hi);
+        fix_atomic_add(acc + threadIdx.x, threadIdx.x ? hi : lo);
+    }
+}
+
+__global__ void k_zero_pair(long long* p)
+{
+    if (threadIdx.x < 2u) p[threadIdx.x] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const float* __restrict__ t,
+                                             const float* __restrict__ mask, T* __restrict__ acc, uint32_t np)
+{
+    __shared__ float part[4];
+    const float v = sse_block(r, t, mask, np, part);
+    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)v);
 }
 
 /* computeSSIMKernel, utils.cu:29-87: sum over masked pixels of the SSIM of the
@@ -636,7 +661,16 @@ __global__ __launch_bounds__(256) void k_ssim(const float* __restrict__ r, const
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)((part[0] + part[1]) + (part[2] + part[3])));
+    const float bsum = (part[0] + part[1]) + (part[2] + part[3]);
+    if constexpr (std::is_same<T, long long>::value) {
+        if (threadIdx.x < 2u) {
+            long long lo, hi;
+            fix_split(bsum, &lo, &hi);
+            fix_atomic_add(acc + threadIdx.x, threadIdx.x ? hi : lo);
+        }
+    } else {
+        if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)bsum);
+    }
 }
 
 /* -------------------------------------------- quantised inference sites --
@@ -852,11 +886,11 @@ static void fill_adam(AdamArgs& A, SadSite* sites, SadAdamState* st, float* cons
     A.sites = sites; A.st = st;
     for (int j = 0; j < 10; ++j) A.g[j] = g[j];
     A.gstride = 1u;
-    A.g64 = nullptr;
+    A.gfix = nullptr;
     A.lr_pos = lr_pos; A.lr_tau = lr_tau; A.lr_radius = lr_radius; A.lr_color = lr_color;
     A.lr_dir = lr_dir; A.lr_aniso = lr_aniso; A.beta1 = beta1; A.beta2 = beta2; A.eps = eps;
-    A.b1c = 1.0f / (1.0f - powf(beta1, (float)t));
-    A.b2c = 1.0f / (1.0f - powf(beta2, (float)t));
+    A.b1c = 1.0f / (1.0f - sad_powi_f(beta1, t));
+    A.b2c = 1.0f / (1.0f - sad_powi_f(beta2, t));
     A.norm = 1.0f / (float)(width * height);
     A.maxx = (float)(width - 1);
     A.maxy = (float)(height - 1);
@@ -892,7 +926,7 @@ int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* t
     if (n == 0) return SADH_OK;
     const dim3 grid(div_up(n, 256u));
     const uint4* nbr = reinterpret_cast<const uint4*>(nbr_lists);
-    hipLaunchKernelGGL(k_tau_list_first<float>, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
+    hipLaunchKernelGGL(k_tau_list_first<const float*>, grid, dim3(256), 0, stream, reinterpret_cast<const uint4*>(c0),
                        reinterpret_cast<const uint4*>(c1), sites, g[2], tmp, raw, n, lambda, cand_width, cand_height,
                        1.0f / fmaxf((float)ds, 1.0f), reinterpret_cast<uint4*>(nbr_lists), 1u);
     hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(256), 0, stream, nbr, (const float*)raw, (const float*)tmp, g[2],
@@ -933,13 +967,12 @@ int launch_update_jobs(BatchJob* jobs, const uint32_t* job_index, const uint32_t
     return launch_status("batch_update_jobs");
 }
 
-/* The list-driven tail on [N][16] gradient lines of type T: pass 1 reads the
- * strided log_tau column (raw saved).  With lists_built (written during the
- * gradient launch) passes 1+2 are one launch into tmp2; otherwise pass 1 writes
- * the neighbour lists into tmp and pass 2 reads them into tmp2.  Passes 3+4 are
- * one launch that feeds Adam, which reads and clears each site's line. */
-template <typename T>
-static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw, float* tmp, float* tmp2,
+/* The list-driven tail on fixed-point gradient lines: pass 1 reads the
+ * log_tau column (raw saved).  With lists_built (written during the gradient
+ * launch) passes 1+2 are one launch into tmp2; otherwise pass 1 writes the
+ * neighbour lists into tmp and pass 2 reads them into tmp2.  Passes 3+4 are one
+ * launch that feeds Adam, which reads and clears each site's line. */
+static void tau_adam_lines(const uint4* u0, const uint4* u1, long long* gacc, float* raw, float* tmp, float* tmp2,
                            float lambda, int cand_width, int cand_height, float inv_ds, SadSite* sites,
                            uint32_t* nbr_lists, bool lists_built, const AdamArgs& A, hipStream_t stream)
 {
@@ -950,15 +983,15 @@ static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw
     const dim3 grid(div_up(n, TB));
     const dim3 grid8(div_up(n, TB / 8u));          /* 8 threads per site */
     uint4* nbr = reinterpret_cast<uint4*>(nbr_lists);
-    const T* g2 = gacc + 2;
+    const FixCol g2{gacc + 4, FIX_GRAD_STRIDE};
     if (lists_built) {
         /* passes 1+2, then 3+4 with Adam: two launches */
-        hipLaunchKernelGGL(k_tau_list_12_built<T>, grid8, dim3(TB), 0, stream, (const uint4*)nbr, g2, tmp2, raw, n,
-                           lambda);
+        hipLaunchKernelGGL(k_tau_list_12_built<FixCol>, grid8, dim3(TB), 0, stream, (const uint4*)nbr, g2, tmp2, raw,
+                           n, lambda);
     } else {
         /* pass 1 writes the lists, so pass 2 cannot share its launch */
-        hipLaunchKernelGGL(k_tau_list_first<T>, grid, dim3(TB), 0, stream, u0, u1, (const SadSite*)sites, g2, tmp,
-                           raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 16u);
+        hipLaunchKernelGGL(k_tau_list_first<FixCol>, grid, dim3(TB), 0, stream, u0, u1, (const SadSite*)sites, g2,
+                           tmp, raw, n, lambda, cand_width, cand_height, inv_ds, nbr, 1u);
         hipLaunchKernelGGL(k_tau_list_pass, grid, dim3(TB), 0, stream, (const uint4*)nbr, (const float*)raw,
                            (const float*)tmp, tmp2, n, lambda);
     }
@@ -966,25 +999,25 @@ static void tau_adam_lines(const uint4* u0, const uint4* u1, T* gacc, float* raw
                        (const float*)tmp2, lambda, A);
 }
 
-/* The same tail on double gradient lines (the fit's cross-tile sums), which
- * always runs from neighbour lists (nbr_lists: 8 uint32 per site). */
-int launch_tau_adam_lines64(const uint32_t* c0, const uint32_t* c1, double* gacc, float* raw, float* tmp, float* tmp2,
+/* The same tail on the fit's fixed-point gradient lines, which always runs
+ * from neighbour lists (nbr_lists: 8 uint32 per site). */
+int launch_tau_adam_fix(const uint32_t* c0, const uint32_t* c1, long long* gacc, float* raw, float* tmp, float* tmp2,
                             float lambda, int cand_width, int cand_height, int ds, SadSite* sites, SadAdamState* st,
                             float lr_pos, float lr_tau, float lr_radius, float lr_color, float lr_dir, float lr_aniso,
                             float beta1, float beta2, float eps, uint32_t t, uint32_t n, int width, int height,
                             hipStream_t stream, uint32_t* nbr_lists, bool lists_built)
 {
     if (n == 0) return SADH_OK;
-    if (!nbr_lists) return fail(SADH_ERR_INVALID, "tau_diffuse_adam(lines64): neighbour lists are required");
+    if (!nbr_lists) return fail(SADH_ERR_INVALID, "tau_diffuse_adam(fix): neighbour lists are required");
     float* g[10] = {};
     AdamArgs A;
     fill_adam(A, sites, st, g, lr_pos, lr_tau, lr_radius, lr_color, lr_dir, lr_aniso, beta1, beta2, eps, t, n, width,
               height);
-    A.g64 = gacc;
+    A.gfix = gacc;
     tau_adam_lines(reinterpret_cast<const uint4*>(c0), reinterpret_cast<const uint4*>(c1), gacc, raw, tmp, tmp2,
                    lambda, cand_width, cand_height, 1.0f / fmaxf((float)ds, 1.0f), sites, nbr_lists, lists_built, A,
                    stream);
-    return launch_status("tau_diffuse_adam(lines64)");
+    return launch_status("tau_diffuse_adam(fix)");
 }
 
 int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
@@ -998,12 +1031,14 @@ int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t
     return launch_status("clear");
 }
 
-/* zeroes n doubles in each array, as 2n floats (the same all-zero bits) */
-int launch_clear64(double* const* arrays, int count_arrays, uint32_t n, hipStream_t stream)
+/* zeroes `words` int64 words, as 2 * words floats (the same all-zero bits); a kernel, not a memset, so the
+ * clear can sit in a captured graph */
+int launch_clear_fix(long long* p, size_t words, hipStream_t stream)
 {
-    float* f[10];
-    for (int j = 0; j < count_arrays; ++j) f[j] = reinterpret_cast<float*>(arrays[j]);
-    return launch_clear(f, count_arrays, 2u * n, stream);
+    if (words == 0) return SADH_OK;
+    if (2u * words > 0xFFFFFFFFu) return fail(SADH_ERR_INVALID, "clear_fix: buffer too large");
+    float* f[1] = {reinterpret_cast<float*>(p)};
+    return launch_clear(f, 1, (uint32_t)(2u * words), stream);
 }
 
 int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* sites, const float* graw,
@@ -1019,7 +1054,7 @@ int launch_tau_diffuse(const uint32_t* c0, const uint32_t* c1, const SadSite* si
 }
 
 template <typename T>
-static int split_impl(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const T* const stats[8],
+static int split_impl(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const T stats[8],
                       uint32_t cur, const float* target, int width, int height, hipStream_t stream)
 {
     if (k == 0) return SADH_OK;
@@ -1035,13 +1070,23 @@ static int split_impl(SadSite* sites, SadAdamState* st, const uint32_t* idx, uin
 int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const float* const stats[8],
                  uint32_t cur, const float* target, int width, int height, hipStream_t stream)
 {
-    return split_impl(sites, st, idx, k, stats, cur, target, width, height, stream);
+    const float* s[8];
+    for (int j = 0; j < 8; ++j) s[j] = stats[j];
+    return split_impl(sites, st, idx, k, s, cur, target, width, height, stream);
 }
 
-int launch_split(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const double* const stats[8],
-                 uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+/* statistics as fixed-point lines (FIX_STAT_STRIDE words per site) */
+static void stat_cols(const long long* stats, FixCol cols[8])
 {
-    return split_impl(sites, st, idx, k, stats, cur, target, width, height, stream);
+    for (int j = 0; j < 8; ++j) cols[j] = FixCol{stats + 2 * j, FIX_STAT_STRIDE};
+}
+
+int launch_split_fix(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint32_t k, const long long* stats,
+                     uint32_t cur, const float* target, int width, int height, hipStream_t stream)
+{
+    FixCol c[8];
+    stat_cols(stats, c);
+    return split_impl(sites, st, idx, k, c, cur, target, width, height, stream);
 }
 
 int launch_sse(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
@@ -1053,12 +1098,12 @@ int launch_sse(const float* rendered, const float* target, const float* mask, fl
     return launch_status("compute_psnr");
 }
 
-int launch_sse64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
-                 hipStream_t stream)
+int launch_sse_fix(const float* rendered, const float* target, const float* mask, long long* acc, int width,
+                   int height, hipStream_t stream)
 {
     uint32_t np = (uint32_t)width * (uint32_t)height;
-    hipLaunchKernelGGL(k_zero_dword, dim3(1), dim3(1), 0, stream, reinterpret_cast<unsigned long long*>(acc));
-    hipLaunchKernelGGL(k_sse<double>, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
+    hipLaunchKernelGGL(k_zero_pair, dim3(1), dim3(64), 0, stream, acc);
+    hipLaunchKernelGGL(k_sse_fix, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
     return launch_status("compute_psnr");
 }
 
@@ -1072,12 +1117,12 @@ int launch_ssim(const float* rendered, const float* target, const float* mask, f
     return launch_status("compute_ssim");
 }
 
-int launch_ssim64(const float* rendered, const float* target, const float* mask, double* acc, int width, int height,
-                  hipStream_t stream)
+int launch_ssim_fix(const float* rendered, const float* target, const float* mask, long long* acc, int width,
+                    int height, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_zero_dword, dim3(1), dim3(1), 0, stream, reinterpret_cast<unsigned long long*>(acc));
+    hipLaunchKernelGGL(k_zero_pair, dim3(1), dim3(64), 0, stream, acc);
     const uint32_t tx = div_up((uint32_t)width, 16u), ty = div_up((uint32_t)height, 16u);
-    hipLaunchKernelGGL(k_ssim<double>, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width,
+    hipLaunchKernelGGL(k_ssim<long long>, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width,
                        height, tx);
     return launch_status("compute_ssim");
 }
@@ -1114,7 +1159,7 @@ int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipSt
 }
 
 template <typename T>
-static int densify_impl(const SadSite* sites, const T* mass, const T* energy, SadPair* pairs, uint32_t n,
+static int densify_impl(const SadSite* sites, T mass, T energy, SadPair* pairs, uint32_t n,
                         float min_mass, float alpha, uint32_t count, hipStream_t stream)
 {
     if (count == 0) return SADH_OK;
@@ -1129,14 +1174,16 @@ int launch_densify_pairs(const SadSite* sites, const float* mass, const float* e
     return densify_impl(sites, mass, energy, pairs, n, min_mass, alpha, count, stream);
 }
 
-int launch_densify_pairs(const SadSite* sites, const double* mass, const double* energy, SadPair* pairs, uint32_t n,
-                         float min_mass, float alpha, uint32_t count, hipStream_t stream)
+int launch_densify_pairs_fix(const SadSite* sites, const long long* stats, SadPair* pairs, uint32_t n, float min_mass,
+                             float alpha, uint32_t count, hipStream_t stream)
 {
-    return densify_impl(sites, mass, energy, pairs, n, min_mass, alpha, count, stream);
+    FixCol c[8];
+    stat_cols(stats, c);
+    return densify_impl(sites, c[0], c[1], pairs, n, min_mass, alpha, count, stream);
 }
 
 template <typename T>
-static int prune_impl(const SadSite* sites, const T* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
+static int prune_impl(const SadSite* sites, T rdelta, SadPair* pairs, uint32_t n, float delta_norm,
                       uint32_t count, hipStream_t stream)
 {
     if (count == 0) return SADH_OK;
@@ -1151,10 +1198,11 @@ int launch_prune_pairs(const SadSite* sites, const float* rdelta, SadPair* pairs
     return prune_impl(sites, rdelta, pairs, n, delta_norm, count, stream);
 }
 
-int launch_prune_pairs(const SadSite* sites, const double* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
-                       uint32_t count, hipStream_t stream)
+/* removal deltas as fixed-point (lo, hi) pairs, one per site */
+int launch_prune_pairs_fix(const SadSite* sites, const long long* rdelta, SadPair* pairs, uint32_t n, float delta_norm,
+                           uint32_t count, hipStream_t stream)
 {
-    return prune_impl(sites, rdelta, pairs, n, delta_norm, count, stream);
+    return prune_impl(sites, FixCol{rdelta, 2u}, pairs, n, delta_norm, count, stream);
 }
 
 int launch_write_indices(const SadPair* sorted, uint32_t* idx, uint32_t count, hipStream_t stream)

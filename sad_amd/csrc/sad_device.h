@@ -36,6 +36,58 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+/* --------------------------------------------------- exact cross-tile sums --
+ * A float partial v joins a cross-tile sum as two int64 words added with
+ * integer atomics: hi = trunc(v), the integer part, and lo = the fraction times
+ * 2^32 rounded to the nearest integer (ties to even).  Integer addition is
+ * associative, so a sum does not depend on the order in which the tiles add
+ * to it.  It is read back as (double)hi + (double)lo * 2^-32, rounded once to
+ * float.
+ *   range:      partials with |v| < 2^62 are held exactly down to 2^-32;
+ *               larger magnitudes saturate at +-2^62 and NaN adds nothing;
+ *   headroom:   |lo| < 2^32 per partial, so the lo word takes 2^31 partials
+ *               before it could overflow, far more than the tiles of an image;
+ *   resolution: 2^-32 per partial, far below what Adam resolves after the
+ *               1/(W*H) gradient normalisation.
+ * A site's values are stored as (lo, hi) pairs: gradient lines of
+ * FIX_GRAD_STRIDE words (values 0-9 in Site field order, then 4 unused words,
+ * 192 B, three 64-byte segments), removal deltas as one pair per site,
+ * densify statistics as lines of FIX_STAT_STRIDE words (8 values). */
+constexpr uint32_t FIX_GRAD_STRIDE = 24u;
+constexpr uint32_t FIX_STAT_STRIDE = 16u;
+
+__device__ __forceinline__ void fix_split(float v, long long* lo, long long* hi)
+{
+    if (!(v == v)) v = 0.0f;
+    v = fminf(fmaxf(v, -0x1p62f), 0x1p62f);
+    const float t = truncf(v);
+    *hi = (long long)t;
+    *lo = (long long)rintf((v - t) * 0x1p32f);
+}
+
+__device__ __forceinline__ float fix_value(long long lo, long long hi)
+{
+    return (float)((double)hi + (double)lo * 0x1p-32);
+}
+
+__device__ __forceinline__ void fix_atomic_add(long long* p, long long v)
+{
+    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+/* One column of (lo, hi) pair lines read as floats: element i of the column is
+ * the pair at p + i * stride (the tau passes, score pairs and split read their
+ * inputs through this, with the float-array code paths unchanged). */
+struct FixCol {
+    const long long* p;
+    uint32_t stride;
+    __device__ __forceinline__ float operator[](uint32_t i) const
+    {
+        const longlong2 w = *reinterpret_cast<const longlong2*>(p + (size_t)i * stride);
+        return fix_value(w.x, w.y);
+    }
+};
+
 /* binary16 -> binary32 with the hardware converter (exact, fp16 denormals on) */
 __device__ __forceinline__ float h2f(uint32_t bits16)
 {

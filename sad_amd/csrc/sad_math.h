@@ -112,6 +112,22 @@ SAD_HD float sad_logf(float x)
     return fmaf(ef, LN2_HI, fmaf(ef, LN2_LO, lm));
 }
 
+/* b^t for an integer t (Adam's bias corrections beta^t): binary powering in
+ * double, rounded once to float.  Double products are exact IEEE roundings on
+ * every target, and their accumulated error (under 2^-48 relative for t < 2^32)
+ * leaves the float result the correctly rounded one except within that
+ * distance of a rounding boundary, so hosts of different libm give the same bits. */
+SAD_HD float sad_powi_f(float b, uint32_t t)
+{
+    double r = 1.0, x = (double)b;
+    while (t) {
+        if (t & 1u) r *= x;
+        x *= x;
+        t >>= 1;
+    }
+    return (float)r;
+}
+
 /* x^a for x > 0 (pow of a positive base), as exp(a * log x). */
 SAD_HD float sad_powf_pos(float x, float a) { return sad_expf(a * sad_logf(x)); }
 
@@ -338,12 +354,29 @@ SAD_HD float sad_packed_key(SadPackedSite p, float ux, float uy, float inv_scale
  * The CPU oracle and the GPU kernels evaluate this one function, so per-term
  * values agree bit for bit and parity is limited to summation order; the
  * function itself is checked against finite differences of the render. */
-/* sqrt(d2) and 1/sqrt(d2) of the derivative terms.  The terms are summed over
- * pixels and compared with a relative tolerance, so a device build may use the
- * hardware reciprocal square root (about 1 ulp) here; the forward pass, whose
- * weights must be bit-exact, never goes through this macro. */
+/* 1/sqrt(x) for normal x > 0 from +, * and fmaf only, so every IEEE binary32
+ * target computes the same bits: the integer seed 0x5F375A86 - (bits(x) >> 1)
+ * (Lomont, "Fast inverse square root", 2003; relative error < 3.5e-2) and three
+ * Newton-Raphson steps y <- y (3/2 - (x/2) y^2), each of which squares the
+ * relative error (3.5e-2 -> 1.8e-3 -> 4.7e-6 -> 3e-11), so the result is within
+ * a few ulp of 1/sqrt(x), the rounding of the last step dominating. */
+SAD_HD float sad_rsqrt_nr(float x)
+{
+    float y = sad_u2f(0x5F375A86u - (sad_f2u(x) >> 1));
+    const float hx = 0.5f * x;
+    y = y * fmaf(-hx, y * y, 1.5f);
+    y = y * fmaf(-hx, y * y, 1.5f);
+    y = y * fmaf(-hx, y * y, 1.5f);
+    return y;
+}
+
+/* sqrt(d2) and 1/sqrt(d2) of the derivative terms (d2 >= 1e-8, so normal):
+ * the Newton reciprocal square root above, the same bits on the GPU and the
+ * CPU, so per-term values are reproducible wherever they are computed.  The
+ * forward pass, whose weights must be bit-exact with the correctly rounded
+ * sqrtf, does not go through this macro. */
 #ifndef SAD_GRAD_SQRT_RSQ
-#define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (sq) = sqrtf(x); (isd) = 1.0f / (sq); } while (0)
+#define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (isd) = sad_rsqrt_nr(x); (sq) = (x) * (isd); } while (0)
 #endif
 
 SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
