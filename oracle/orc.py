@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
         _lib.orc_fit.restype = I32
         _lib.orc_init_gradient_weighted.restype = U32
         _lib.orc_init_grad_eval.restype = I32
+        _lib.orc_fix_value.restype = F32
+        _lib.orc_sse_fix.restype = F32
+        _lib.orc_simulate_final.restype = I32
     return _lib
 
 
@@ -220,6 +223,70 @@ def render_backward(cand0, cand1, grad_out, sites, iss, width, height):
     return acc, ab
 
 
+# ------------------------------------------------- the library's exact sums
+def fix_split(v):
+    """float32 -> (lo, hi) int64 words of the library's fixed point"""
+    lo, hi = C.c_int64(), C.c_int64()
+    lib().orc_fix_split(F32(float(v)), C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def fix_values(pairs):
+    """int64 [..., 2] (lo, hi) pairs -> float32 [...]"""
+    p = np.ascontiguousarray(pairs, np.int64)
+    out = np.zeros(p.shape[:-1], np.float32)
+    lib().orc_fix_values(_p(p), _p(out), I64(out.size))
+    return out
+
+
+# resolution of the library's fixed point: a sum of P float partials is within P * FIX_HALF_STEP of exact
+FIX_HALF_STEP = 2.0 ** -41
+
+
+def gradients_fix(cand0, cand1, target, mask, sites, iss, width, height, compute_removal=False, cand_w=None,
+                  cand_h=None, rendered=None, ssim_weight=0.0):
+    """The tiled gradient summed as the HIP library sums it: (grad [N,10] float32, removal [N] float32,
+    partials per site [N] int32)."""
+    c0, c1, t, m, s = _u32(cand0), _u32(cand1), _f32(target), _f32(mask), _f32(sites)
+    n = s.shape[0]
+    g = np.zeros((n, 10, 2), np.int64)
+    r = np.zeros((n, 2), np.int64)
+    parts = np.zeros(n, np.int32)
+    ren = _f32(rendered) if ssim_weight > 0.0 else None
+    lib().orc_gradients_fix(_p(c0), _p(c1), _p(t), _p(m), _p(s), F32(iss), U32(n), I32(int(compute_removal)),
+                            I32(width), I32(height), I32(cand_w or width), I32(cand_h or height), _p(ren),
+                            F32(ssim_weight), _p(g), _p(r), _p(parts))
+    return fix_values(g), fix_values(r), parts
+
+
+def render_backward_fix(cand0, cand1, grad_out, sites, iss, width, height):
+    """-> (grad [N,10] float32, partials per site [N] int32)"""
+    c0, c1, go, s = _u32(cand0), _u32(cand1), _f32(grad_out), _f32(sites)
+    n = s.shape[0]
+    g = np.zeros((n, 10, 2), np.int64)
+    parts = np.zeros(n, np.int32)
+    lib().orc_render_backward_fix(_p(c0), _p(c1), _p(go), _p(s), F32(iss), U32(n), I32(width), I32(height),
+                                  I32(width), I32(height), _p(g), _p(parts))
+    return fix_values(g), parts
+
+
+def site_stats_fix(cand0, cand1, target, mask, sites, iss, width, height):
+    """-> (stats [N,8] float32, partials per site [N] int32)"""
+    c0, c1, t, m, s = _u32(cand0), _u32(cand1), _f32(target), _f32(mask), _f32(sites)
+    n = s.shape[0]
+    g = np.zeros((n, 8, 2), np.int64)
+    parts = np.zeros(n, np.int32)
+    lib().orc_site_stats_fix(_p(c0), _p(c1), _p(t), _p(m), _p(s), F32(iss), U32(n), I32(width), I32(height),
+                             I32(width), I32(height), _p(g), _p(parts))
+    return fix_values(g), parts
+
+
+def sse_fix(rendered, target, mask):
+    r, t, m = _f32(rendered), _f32(target), _f32(mask)
+    h, w = m.shape
+    return lib().orc_sse_fix(_p(r), _p(t), _p(m), I32(w), I32(h))
+
+
 def tau_diffuse(cand0, cand1, sites, graw, gin, lam, width, height, downscale=1):
     c0, c1, s, gr, gi = _u32(cand0), _u32(cand1), _f32(sites), _f32(graw), _f32(gin)
     out = np.zeros_like(gi)
@@ -292,6 +359,25 @@ def _cfg(opts: dict) -> OrcFitConfig:
     for k, _ in OrcFitConfig._fields_:
         setattr(c, k, opts[k])
     return c
+
+
+def simulate_final_sites(init_sites, max_sites, iters, densify_enabled, d_start, d_end, d_freq, d_pct,
+                         prune_during, p_start, p_end, p_freq, p_pct, max_split=65536):
+    """simulateFinalSites (sad_cuda.cu:904-960): the active-site count a schedule ends with"""
+    return int(lib().orc_simulate_final(I32(init_sites), I32(max_sites), I32(iters), I32(int(densify_enabled)),
+                                        I32(d_start), I32(d_end), I32(d_freq), F32(d_pct), I32(int(prune_during)),
+                                        I32(p_start), I32(p_end), I32(p_freq), F32(p_pct), I32(max_split)))
+
+
+def solve_target_bpp(target_bpp, width, height, init_sites, max_sites, iters, densify_enabled, d_start, d_end,
+                     d_freq, base_d, prune_during, p_start, p_end, p_freq, base_p, max_split=65536):
+    """solveTargetBpp (sad_cuda.cu:962-1066) -> (densify pct, prune pct, simulated final sites)"""
+    d, p, n = C.c_float(), C.c_float(), C.c_int()
+    lib().orc_solve_target_bpp(F32(target_bpp), I32(width), I32(height), I32(init_sites), I32(max_sites), I32(iters),
+                               I32(int(densify_enabled)), I32(d_start), I32(d_end), I32(d_freq), F32(base_d),
+                               I32(int(prune_during)), I32(p_start), I32(p_end), I32(p_freq), F32(base_p),
+                               I32(max_split), C.byref(d), C.byref(p), C.byref(n))
+    return d.value, p.value, n.value
 
 
 def fit_plan(opts: dict, width: int, height: int) -> dict:

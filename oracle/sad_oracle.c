@@ -539,8 +539,8 @@ ORC_API void orc_render_backward(const uint32_t* cand0, const uint32_t* cand1, c
  * is the running float sum from 0 of its chunks in order.
  *
  * Fixed point: a partial v becomes hi = trunc(v), lo = round-half-even of
- * (v - trunc(v)) * 2^32 (|v| saturated at 2^62, NaN taken as 0); sums of hi and
- * of lo are int64; the value is (float)((double)hi + (double)lo * 2^-32). */
+ * (v - trunc(v)) * 2^40 (|v| saturated at 2^62, NaN taken as 0); sums of hi and
+ * of lo are int64; the value is (float)((double)hi + (double)lo * 2^-40). */
 #define ORC_CHUNK 24
 #define ORC_LANES 4
 #define ORC_TILE_ENTRIES (256 * SAD_K)
@@ -552,10 +552,10 @@ ORC_API void orc_fix_split(float v, int64_t* lo, int64_t* hi)
     if (v < -0x1p62f) v = -0x1p62f;
     float t = truncf(v);
     *hi = (int64_t)t;
-    *lo = (int64_t)nearbyintf((v - t) * 0x1p32f);
+    *lo = (int64_t)nearbyintf((v - t) * 0x1p40f);
 }
 
-ORC_API float orc_fix_value(int64_t lo, int64_t hi) { return (float)((double)hi + (double)lo * 0x1p-32); }
+ORC_API float orc_fix_value(int64_t lo, int64_t hi) { return (float)((double)hi + (double)lo * 0x1p-40); }
 
 static void orc_fix_add(int64_t* pair, float v)
 {
@@ -594,6 +594,7 @@ typedef struct OrcTileJob {
     int64_t* out;              /* [n][stride] (lo, hi) pairs of value j at 2 j */
     int stride;
     int64_t* rout;             /* removal: [n][2] */
+    int32_t* parts;            /* nullable: partials (tiles) added per site */
 } OrcTileJob;
 
 /* the pixels of one tile: forward weights, dL/dpred, the summed pairs */
@@ -734,6 +735,7 @@ static void orc_tile(const OrcTileJob* J, int tile, int tiles_x, OrcTilePixel* p
             if (J->removal && j == 10) orc_fix_add(&J->rout[2 * (size_t)id], total[j]);
             else orc_fix_add(&J->out[(size_t)id * J->stride + 2 * j], total[j]);
         }
+        if (J->parts) __atomic_fetch_add(&J->parts[id], 1, __ATOMIC_RELAXED);
     }
 }
 
@@ -765,21 +767,21 @@ static void orc_tiles_run(const OrcTileJob* J)
 ORC_API void orc_gradients_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                                const SadSite* sites, float iss, uint32_t n, int compute_removal, int width,
                                int height, int cand_w, int cand_h, const float* rendered, float ssim_weight,
-                               int64_t* gfix, int64_t* rfix)
+                               int64_t* gfix, int64_t* rfix, int32_t* parts)
 {
     OrcTileJob J = {0, compute_removal ? 1 : 0, cand0, cand1, target, mask, NULL, rendered,
                     ssim_weight > 0.0f ? ssim_weight : 0.0f, sites, iss, n, width, height, cand_w, cand_h,
-                    gfix, 20, rfix};
+                    gfix, 20, rfix, parts};
     orc_tiles_run(&J);
 }
 
 /* render_sad_backward_kernel (sad_bindings.cu:100-207) with the library's sums, into gfix [n][20] */
 ORC_API void orc_render_backward_fix(const uint32_t* cand0, const uint32_t* cand1, const float* grad_out,
                                      const SadSite* sites, float iss, uint32_t n, int width, int height,
-                                     int cand_w, int cand_h, int64_t* gfix)
+                                     int cand_w, int cand_h, int64_t* gfix, int32_t* parts)
 {
     OrcTileJob J = {1, 0, cand0, cand1, NULL, NULL, grad_out, NULL, 0.0f, sites, iss, n, width, height, cand_w,
-                    cand_h, gfix, 20, NULL};
+                    cand_h, gfix, 20, NULL, parts};
     orc_tiles_run(&J);
 }
 
@@ -787,10 +789,10 @@ ORC_API void orc_render_backward_fix(const uint32_t* cand0, const uint32_t* cand
  * (lo, hi) of mass, energy, w.err, w.err.x, w.err.y, w.err.xx, w.err.xy, w.err.yy */
 ORC_API void orc_site_stats_fix(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                                 const SadSite* sites, float iss, uint32_t n, int width, int height, int cand_w,
-                                int cand_h, int64_t* sfix)
+                                int cand_h, int64_t* sfix, int32_t* parts)
 {
     OrcTileJob J = {2, 0, cand0, cand1, target, mask, NULL, NULL, 0.0f, sites, iss, n, width, height, cand_w, cand_h,
-                    sfix, 16, NULL};
+                    sfix, 16, NULL, parts};
     orc_tiles_run(&J);
 }
 

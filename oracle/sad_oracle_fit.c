@@ -53,9 +53,9 @@ void orc_split_sites(SadSite*, SadAdamState*, const uint32_t*, uint32_t, const f
 void orc_hilbert_pairs(const SadSite*, SadPair*, uint32_t, uint32_t, int, int, int);
 void orc_hilbert_order(const SadPair*, uint32_t*, uint32_t*, uint32_t);
 void orc_gradients_fix(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
-                       int, int, int, int, int, const float*, float, int64_t*, int64_t*);
+                       int, int, int, int, int, const float*, float, int64_t*, int64_t*, int32_t*);
 void orc_site_stats_fix(const uint32_t*, const uint32_t*, const float*, const float*, const SadSite*, float, uint32_t,
-                        int, int, int, int, int64_t*);
+                        int, int, int, int, int64_t*, int32_t*);
 float orc_sse_fix(const float*, const float*, const float*, int, int);
 float orc_fix_value(int64_t, int64_t);
 double orc_ssim_sum(const float*, const float*, const float*, int, int, float*);
@@ -100,7 +100,7 @@ static int imax(int a, int b) { return a > b ? a : b; }
 static int imin(int a, int b) { return a < b ? a : b; }
 
 /* simulateFinalSites, sad_cuda.cu:904-960 */
-static int orc_simulate_final(int init_sites, int max_sites, int iters, int densify_enabled,
+ORC_API int orc_simulate_final(int init_sites, int max_sites, int iters, int densify_enabled,
                               int d_start, int d_end, int d_freq, float d_pct, int prune_during,
                               int p_start, int p_end, int p_freq, float p_pct, int max_split)
 {
@@ -350,7 +350,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
         if (densify && cfg->densify_enabled && needs_pairs) {
             n_dens++;
             memset(sfix, 0, sizeof(int64_t) * 16 * n);
-            orc_site_stats_fix(cA0, cA1, target, mask, sites, iss, n, width, height, cw, ch, sfix);
+            orc_site_stats_fix(cA0, cA1, target, mask, sites, iss, n, width, height, cw, ch, sfix, NULL);
             for (uint32_t i = 0; i < n; ++i)
                 for (int j = 0; j < 8; ++j) st[j][i] = orc_fix_value(sfix[16 * (size_t)i + 2 * j], sfix[16 * (size_t)i + 2 * j + 1]);
             orc_densify_pairs(sites, st[0], st[1], pairs, n, 1.0f, cfg->densify_score_alpha, (uint32_t)pairs_count);
@@ -378,7 +378,7 @@ ORC_API int orc_fit(const OrcFitConfig* cfg, const float* target, const float* m
             orc_render(gA0, gA1, render, sites, iss, n, width, height, cw, ch);
         }
         orc_gradients_fix(gA0, gA1, target, mask, sites, iss, n, prune, width, height, cw, ch,
-                          cfg->ssim_weight > 0.0f ? render : NULL, cfg->ssim_weight, gfix, prune ? rfix : NULL);
+                          cfg->ssim_weight > 0.0f ? render : NULL, cfg->ssim_weight, gfix, prune ? rfix : NULL, NULL);
         for (uint32_t i = 0; i < n; ++i) {
             for (int j = 0; j < 10; ++j)
                 grads[j][i] += orc_fix_value(gfix[20 * (size_t)i + 2 * j], gfix[20 * (size_t)i + 2 * j + 1]);

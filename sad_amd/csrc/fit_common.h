@@ -150,7 +150,7 @@ int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, c
 int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
 
 /* a fixed-point (lo, hi) sum read back on the host, with the device's rounding (fix_value) */
-inline float fix_value_host(const long long w[2]) { return (float)((double)w[1] + (double)w[0] * 0x1p-32); }
+inline float fix_value_host(const long long w[2]) { return (float)((double)w[1] + (double)w[0] * 0x1p-40); }
 
 /* capacity and densify/prune percentiles of a fit (sad_cuda.cu:904-1098) */
 int plan(const SadhFitOptions& o, int width, int height, float* d_pct, float* p_pct, int* cap, int* pairs);

@@ -53,7 +53,7 @@
  */
 #include <type_traits>
 
-#include "batch.h"
+#include "fit_common.h"
 
 namespace sadh {
 
@@ -779,6 +779,79 @@ int launch_tau_heatmap(const uint32_t* cand0, const uint32_t* cand1, float* out,
     return launch_status("render_tau_heatmap");
 }
 
+/* ----------------------------------------- float outputs via exact lines --
+ * The reference-interface entry points write float arrays.  They accumulate
+ * into fixed-point lines in a per-stream workspace (stream_workspace), cleared
+ * by a kernel first, then k_fix_finalize adds (or, for the statistics, stores)
+ * each value into the caller's floats: the same bits whatever the order of
+ * the tiles, and one line-shaped atomic target per (tile, site) instead of
+ * ten scattered float arrays. */
+struct FixOut {
+    float* g[10];              /* value j of site i at g[j] + i * gstride */
+    uint32_t gstride, nv, lstride, assign;
+    float* removal;            /* nullable: removal delta of site i at removal[i] */
+    const long long* lines;    /* (lo, hi) of value j of site i at lines + i * lstride + 2 j */
+    const long long* rlines;   /* (lo, hi) of the removal delta of site i at rlines + 2 i */
+    uint32_t n;
+};
+
+__global__ __launch_bounds__(256) void k_fix_finalize(FixOut F)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= F.n) return;
+    const longlong2* L = reinterpret_cast<const longlong2*>(F.lines + (size_t)i * F.lstride);
+#pragma unroll
+    for (uint32_t j = 0; j < 10u; ++j) {
+        if (j >= F.nv) break;
+        const longlong2 w = L[j];
+        const float v = fix_value(w.x, w.y);
+        float* d = F.g[j] + (size_t)i * F.gstride;
+        *d = F.assign ? v : *d + v;
+    }
+    if (F.removal) {
+        const longlong2 w = reinterpret_cast<const longlong2*>(F.rlines)[i];
+        F.removal[i] += fix_value(w.x, w.y);
+    }
+}
+
+static int launch_fix_finalize(const FixOut& F, hipStream_t stream)
+{
+    if (F.n == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_fix_finalize, dim3(div_up(F.n, 256u)), dim3(256), 0, stream, F);
+    return launch_status("finalize_sums");
+}
+
+/* workspace for n sites of `line_words` words, plus (removal) one pair each; cleared */
+static int fix_workspace(hipStream_t stream, uint32_t n, uint32_t line_words, bool removal, long long** lines,
+                         long long** rlines)
+{
+    const size_t words = (size_t)n * line_words + (removal ? 2u * (size_t)n : 0u);
+    void* p = nullptr;
+    SAD_TRY(stream_workspace(stream, std::max<size_t>(words, 1) * sizeof(long long), &p));
+    *lines = static_cast<long long*>(p);
+    *rlines = removal ? *lines + (size_t)n * line_words : nullptr;
+    return launch_clear_fix(*lines, words, stream);
+}
+
+/* the tiled gradient into float destinations: g[j][i * gstride] += value j of site i (and removal[i] += the
+ * removal delta) */
+static int gradients_to_floats(PixelArgs& A, float* const g[10], uint32_t gstride, float* removal,
+                               bool compute_removal, hipStream_t stream)
+{
+    long long *lines, *rlines;
+    SAD_TRY(fix_workspace(stream, A.n, FIX_GRAD_STRIDE, compute_removal, &lines, &rlines));
+    A.gfix = lines;
+    A.rfix = rlines;
+    if (compute_removal) SAD_TRY((launch_pixel<MODE_GRAD, true, DST_FIX>(A, stream, "compute_gradients_tiled")));
+    else SAD_TRY((launch_pixel<MODE_GRAD, false, DST_FIX>(A, stream, "compute_gradients_tiled")));
+    FixOut F = {};
+    for (int j = 0; j < 10; ++j) F.g[j] = g[j];
+    F.gstride = gstride; F.nv = 10; F.lstride = FIX_GRAD_STRIDE; F.assign = 0;
+    F.removal = compute_removal ? removal : nullptr;
+    F.lines = lines; F.rlines = rlines; F.n = A.n;
+    return launch_fix_finalize(F, stream);
+}
+
 int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                      float* const g[10], const SadSite* sites, float iss, uint32_t n, float* removal,
                      bool compute_removal, int width, int height, int cw, int ch, hipStream_t stream,
@@ -787,15 +860,12 @@ int launch_gradients(const uint32_t* cand0, const uint32_t* cand1, const float* 
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
     A.rendered = rendered; A.ssim_weight = ssim_weight;
-    for (int j = 0; j < 10; ++j) A.g[j] = g[j];
-    A.gstride = 1u;
-    A.removal = removal; A.iss = iss; A.n = n;
+    A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    if (compute_removal) return launch_pixel<MODE_GRAD, true>(A, stream, "compute_gradients_tiled");
-    return launch_pixel<MODE_GRAD, false>(A, stream, "compute_gradients_tiled");
+    return gradients_to_floats(A, g, 1u, removal, compute_removal, stream);
 }
 
-/* gradients into AoS [N][16] lines (values 0-9 in Site field order) */
+/* gradients into AoS [N][16] float lines (values 0-9 in Site field order) */
 int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                          float* gacc, const SadSite* sites, float iss, uint32_t n, float* removal, bool compute_removal,
                          int width, int height, int cw, int ch, hipStream_t stream, const float* rendered,
@@ -804,12 +874,11 @@ int launch_gradients_aos(const uint32_t* cand0, const uint32_t* cand1, const flo
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
     A.rendered = rendered; A.ssim_weight = ssim_weight;
-    A.gline = gacc;
-    A.gstride = 16u;
-    A.removal = removal; A.iss = iss; A.n = n;
+    A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    if (compute_removal) return launch_pixel<MODE_GRAD, true, DST_LINES>(A, stream, "compute_gradients_tiled(aos)");
-    return launch_pixel<MODE_GRAD, false, DST_LINES>(A, stream, "compute_gradients_tiled(aos)");
+    float* g[10];
+    for (int j = 0; j < 10; ++j) g[j] = gacc + j;
+    return gradients_to_floats(A, g, 16u, removal, compute_removal, stream);
 }
 
 /* the fit's gradients: fixed-point (lo, hi) lines and removal pairs */
@@ -849,16 +918,24 @@ int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_
     return launch_status("compute_gradients_batch");
 }
 
+/* computeSiteStatsSimpleKernel's float outputs (utils.cu:106-231): the statistics of site i stored into
+ * st[j][i] (the reference clears them first) */
 int launch_site_stats(const uint32_t* cand0, const uint32_t* cand1, const float* target, const float* mask,
                       const SadSite* sites, float iss, uint32_t n, float* const st[8], int width, int height,
                       int cw, int ch, hipStream_t stream)
 {
+    long long *lines, *unused;
+    SAD_TRY(fix_workspace(stream, n, FIX_STAT_STRIDE, false, &lines, &unused));
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
-    for (int j = 0; j < 8; ++j) A.g[j] = st[j];
-    A.gstride = 1u;
+    A.sfix = lines;
     A.iss = iss; A.n = n; A.width = width; A.height = height; A.cw = cw; A.ch = ch;
-    return launch_pixel<MODE_STATS, false>(A, stream, "compute_site_stats");
+    SAD_TRY((launch_pixel<MODE_STATS, false, DST_FIXSTATS>(A, stream, "compute_site_stats")));
+    FixOut F = {};
+    for (int j = 0; j < 8; ++j) F.g[j] = st[j];
+    F.gstride = 1u; F.nv = 8; F.lstride = FIX_STAT_STRIDE; F.assign = 1;
+    F.lines = lines; F.n = n;
+    return launch_fix_finalize(F, stream);
 }
 
 /* the fit's densify statistics, in fixed-point (lo, hi) lines of FIX_STAT_STRIDE words */
@@ -873,7 +950,6 @@ int launch_site_stats_fix(const uint32_t* cand0, const uint32_t* cand1, const fl
     return launch_pixel<MODE_STATS, false, DST_FIXSTATS>(A, stream, "compute_site_stats(fix)");
 }
 
-int launch_clear(float* const* arrays, int count_arrays, uint32_t n, hipStream_t stream);
 
 }  // namespace sadh
 
@@ -917,13 +993,21 @@ int sadh_render_backward(const uint32_t* cand0, const uint32_t* cand1, const flo
 {
     SADH_REQUIRE(cand0 && cand1 && grad_output && grad_sites && sites, "render_backward: null pointer");
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "render_backward: bad sizes");
+    const hipStream_t S = to_hip(stream);
+    long long *lines, *unused;
+    SAD_TRY(fix_workspace(S, site_count, FIX_GRAD_STRIDE, false, &lines, &unused));
     PixelArgs A = {};
     A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.grad_out = grad_output;
-    A.gline = grad_sites;
-    A.gstride = 10u;
+    A.gfix = lines;
     A.iss = inv_scale_sq; A.n = site_count; A.width = width; A.height = height;
     A.cw = cand_width; A.ch = cand_height;
-    return launch_pixel<MODE_BWD, false, DST_LINES>(A, to_hip(stream), "render_backward");
+    SAD_TRY((launch_pixel<MODE_BWD, false, DST_FIX>(A, S, "render_backward")));
+    /* grad_sites [N][10] in Site field order, added to */
+    FixOut F = {};
+    for (int j = 0; j < 10; ++j) F.g[j] = grad_sites + j;
+    F.gstride = 10u; F.nv = 10; F.lstride = FIX_GRAD_STRIDE; F.assign = 0;
+    F.lines = lines; F.n = site_count;
+    return launch_fix_finalize(F, S);
 }
 
 int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, const float* target,
@@ -972,8 +1056,6 @@ int sadh_compute_site_stats_simple(const uint32_t* cand0, const uint32_t* cand1,
     for (int j = 0; j < 8; ++j) SADH_REQUIRE(st[j], "compute_site_stats_simple: null stats array");
     SADH_REQUIRE(cand0 && cand1 && target && mask && sites, "compute_site_stats_simple: null pointer");
     SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0, "compute_site_stats_simple: bad sizes");
-    int rc = launch_clear(st, 8, site_count, to_hip(stream));
-    if (rc) return rc;
     return launch_site_stats(cand0, cand1, target, mask, sites, inv_scale_sq, site_count, st, width, height,
                              cand_width, cand_height, to_hip(stream));
 }

@@ -9,7 +9,7 @@
 #include <algorithm>
 #include <type_traits>
 
-#include "batch.h"
+#include "fit_common.h"
 
 namespace sadh {
 
@@ -1089,12 +1089,20 @@ int launch_split_fix(SadSite* sites, SadAdamState* st, const uint32_t* idx, uint
     return split_impl(sites, st, idx, k, c, cur, target, width, height, stream);
 }
 
+__global__ void k_fix_store(const long long* __restrict__ w, float* __restrict__ out)
+{
+    if (threadIdx.x == 0) *out = fix_value(w[0], w[1]);
+}
+
+/* launchComputePSNR's float accumulator (utils.cu:258-276): the block sums meet in a fixed-point pair of a
+ * per-stream workspace, then the float is stored, so the value does not depend on the blocks' order */
 int launch_sse(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
                hipStream_t stream)
 {
-    uint32_t np = (uint32_t)width * (uint32_t)height;
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
-    hipLaunchKernelGGL(k_sse<float>, dim3(div_up(np, 256u)), dim3(256), 0, stream, rendered, target, mask, acc, np);
+    void* w = nullptr;
+    SAD_TRY(stream_workspace(stream, 2 * sizeof(long long), &w));
+    SAD_TRY(launch_sse_fix(rendered, target, mask, static_cast<long long*>(w), width, height, stream));
+    hipLaunchKernelGGL(k_fix_store, dim3(1), dim3(64), 0, stream, static_cast<const long long*>(w), acc);
     return launch_status("compute_psnr");
 }
 
@@ -1107,13 +1115,14 @@ int launch_sse_fix(const float* rendered, const float* target, const float* mask
     return launch_status("compute_psnr");
 }
 
+/* launchComputeSSIM's float accumulator (utils.cu:278-296), through a fixed-point pair as launch_sse */
 int launch_ssim(const float* rendered, const float* target, const float* mask, float* acc, int width, int height,
                 hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, reinterpret_cast<uint32_t*>(acc));
-    const uint32_t tx = div_up((uint32_t)width, 16u), ty = div_up((uint32_t)height, 16u);
-    hipLaunchKernelGGL(k_ssim<float>, dim3(tx * ty), dim3(256), 0, stream, rendered, target, mask, acc, width, height,
-                       tx);
+    void* w = nullptr;
+    SAD_TRY(stream_workspace(stream, 2 * sizeof(long long), &w));
+    SAD_TRY(launch_ssim_fix(rendered, target, mask, static_cast<long long*>(w), width, height, stream));
+    hipLaunchKernelGGL(k_fix_store, dim3(1), dim3(64), 0, stream, static_cast<const long long*>(w), acc);
     return launch_status("compute_ssim");
 }
 

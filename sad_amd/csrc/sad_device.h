@@ -27,6 +27,10 @@ int launch_status(const char* what);
 inline hipStream_t to_hip(sadh_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 
+/* A device workspace of at least `bytes` for calls on `stream` (scratch.cpp): owned by the library, reused by
+ * later calls on the same stream and device. */
+int stream_workspace(hipStream_t stream, size_t bytes, void** out);
+
 /* ------------------------------------------------------ fast arithmetic --
  * Hardware transcendental approximations (v_exp_f32, v_sqrt_f32, v_rsq_f32,
  * v_rcp_f32; about 1 ulp) for the render and gradient kernels, whose parity bar
@@ -39,16 +43,18 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 /* --------------------------------------------------- exact cross-tile sums --
  * A float partial v joins a cross-tile sum as two int64 words added with
  * integer atomics: hi = trunc(v), the integer part, and lo = the fraction times
- * 2^32 rounded to the nearest integer (ties to even).  Integer addition is
+ * 2^40 rounded to the nearest integer (ties to even).  Integer addition is
  * associative, so a sum does not depend on the order in which the tiles add
- * to it.  It is read back as (double)hi + (double)lo * 2^-32, rounded once to
+ * to it.  It is read back as (double)hi + (double)lo * 2^-40, rounded once to
  * float.
- *   range:      partials with |v| < 2^62 are held exactly down to 2^-32;
+ *   range:      partials with |v| < 2^62 are held exactly down to 2^-40;
  *               larger magnitudes saturate at +-2^62 and NaN adds nothing;
- *   headroom:   |lo| < 2^32 per partial, so the lo word takes 2^31 partials
- *               before it could overflow, far more than the tiles of an image;
- *   resolution: 2^-32 per partial, far below what Adam resolves after the
- *               1/(W*H) gradient normalisation.
+ *   headroom:   |lo| < 2^40 per partial, so the lo word takes 2^23 partials
+ *               (tiles of one site in one launch) before it could overflow:
+ *               images up to 2^31 pixels;
+ *   resolution: 2^-40 (9.1e-13) per partial, so a sum of P partials is within
+ *               P * 2^-41 of their exact sum; far below what Adam resolves
+ *               after the 1/(W*H) gradient normalisation.
  * A site's values are stored as (lo, hi) pairs: gradient lines of
  * FIX_GRAD_STRIDE words (values 0-9 in Site field order, then 4 unused words,
  * 192 B, three 64-byte segments), removal deltas as one pair per site,
@@ -62,12 +68,12 @@ __device__ __forceinline__ void fix_split(float v, long long* lo, long long* hi)
     v = fminf(fmaxf(v, -0x1p62f), 0x1p62f);
     const float t = truncf(v);
     *hi = (long long)t;
-    *lo = (long long)rintf((v - t) * 0x1p32f);
+    *lo = (long long)rintf((v - t) * 0x1p40f);
 }
 
 __device__ __forceinline__ float fix_value(long long lo, long long hi)
 {
-    return (float)((double)hi + (double)lo * 0x1p-32);
+    return (float)((double)hi + (double)lo * 0x1p-40);
 }
 
 __device__ __forceinline__ void fix_atomic_add(long long* p, long long v)

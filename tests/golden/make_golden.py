@@ -17,7 +17,13 @@ Fixtures (one .npz each, compressed):
   sort_pairs.npz    4096 densify score pairs, sorted (4 passes), and a 16-bit
                     key set sorted with the 2-pass rule;
   adam_step.npz     one Adam step (t = 7) on 256 sites with seeded state and
-                    gradients: new sites and state.
+                    gradients: new sites and state;
+  fit1024_full.npz  the full 1024x1024 fit (16,384 initial sites, densify and
+                    prune every 100 iterations, 262,144-site buffer, 1.0 bpp,
+                    2,000 iterations): SHA-256 of the final [262144, 10] sites,
+                    their first 256 rows, the PSNR logged every 100 iterations
+                    and the schedule counts.  Written only on request
+                    (`make_golden.py fit1024`): the oracle needs a few minutes.
 """
 from __future__ import annotations
 
@@ -88,9 +94,29 @@ def adam_inputs():
 ADAM_LRS = (0.025, 0.01, 0.01, 0.01, 0.0025, 0.025)
 
 
+FIT1024_LOG_FREQ = 100
+
+
+def fit1024_full(orc) -> dict:
+    """The oracle's full 1024x1024 fit, as stored in fit1024_full.npz."""
+    from sad_amd import config, synth
+    tgt, sites, over = synth.fit_workload("fit1024")
+    opts = config.fit_options(**dict(over, log_freq=FIT1024_LOG_FREQ))
+    r = orc.fit(tgt, sites, opts)
+    return dict(sites_sha=np.array(digest(r["sites"])), sites_head=r["sites"][:256],
+                log_psnr=np.asarray(r["log_psnr"], np.float32), final_psnr=np.float32(r["final_psnr"]),
+                counts=np.array([r["n_densify"], r["n_prune"], r["actual_sites"], r["active_estimate"],
+                                 r["n_cand_updates"]], np.int64))
+
+
 def main() -> None:
     from oracle import orc
     orc.lib()
+    if sys.argv[1:] == ["fit1024"]:
+        orc.set_threads(os.cpu_count() or 1)
+        np.savez_compressed(os.path.join(HERE, "fit1024_full.npz"), **fit1024_full(orc))
+        print("fit1024_full.npz", os.path.getsize(os.path.join(HERE, "fit1024_full.npz")), "bytes")
+        return
     _, sites = fit256_scene()
     w = h = 256
     c0, c1 = orc.init_candidates(len(sites), 0, False, w, h)

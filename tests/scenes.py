@@ -7,6 +7,25 @@ import numpy as np
 from sad_amd import synth
 
 
+def same_bits(a, b) -> bool:
+    """float32 arrays equal bit for bit"""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+SCHEDULE = ("n_densify", "n_prune", "actual_sites", "active_estimate", "n_cand_updates")
+
+
+def assert_same_fit(g: dict, r: dict) -> None:
+    """two fits (the GPU's, the oracle's or a repeat) ended with the same bits: sites, PSNR log, schedule"""
+    assert tuple(g[k] for k in SCHEDULE) == tuple(r[k] for k in SCHEDULE)
+    bad = int(np.count_nonzero(np.asarray(g["sites"], np.float32).view(np.uint32) !=
+                               np.asarray(r["sites"], np.float32).view(np.uint32)))
+    assert bad == 0, f"{bad} site values differ"
+    assert same_bits(g["log_psnr"], r["log_psnr"]), (g["log_psnr"], r["log_psnr"])
+
+
 def iss_of(w, h):
     m = np.float32(max(w, h))
     return float(np.float32(1.0) / (m * m))

@@ -124,9 +124,13 @@ def test_gradient_and_adam(gpu, orc, c3, field):
     rd = gpu.zeros((N,))
     gpu.call("sadh_compute_gradients_tiled", field["a0"], field["a1"], gt, gt, gm, *grads, c3["gs"], c3["iss"], N, rd,
              0, 0.0, W, H, W, H)
-    acc, ab, _, _ = orc.gradients(c0, c1, c3["target"], np.ones((H, W), np.float32), c3["sites"], c3["iss"], W, H)
+    ones = np.ones((H, W), np.float32)
+    acc, ab, _, _ = orc.gradients(c0, c1, c3["target"], ones, c3["sites"], c3["iss"], W, H)
     got = np.stack([gpu.get(g) for g in grads], 1)
-    assert Rel(got, acc, ab) <= 1.0
+    gf, _, parts = orc.gradients_fix(c0, c1, c3["target"], ones, c3["sites"], c3["iss"], W, H)
+    assert Rel(got, acc, ab, parts[:, None] * orc.FIX_HALF_STEP) <= 1.0
+    # the library's sums, reproduced by the oracle's tile-ordered fixed point
+    assert np.array_equal(got.view(np.uint32), gf.view(np.uint32))
     # Adam + clear from these gradients, against the oracle on the same float32 gradients
     lrs = (0.025, 0.01, 0.01, 0.01, 0.0025, 0.025)
     state = gpu.zeros((N, 20))

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from tests.scenes import converged_field, hilbert_tables, iss_of, random_sites
+from oracle import orc as orc_mod
 from tests.tolerance import Rel
 
 pytestmark = pytest.mark.gpu
@@ -188,8 +189,22 @@ def test_render_edge_cases(gpu, orc):
     assert np.abs(gpu.get(out) - ref).max() <= 1e-5
 
 
-def _rel_ok(got, ref, ab):
-    return Rel(got, ref, ab)
+def _rel_ok(got, ref, ab, parts=None):
+    """Rel with the fixed-point resolution of `parts` partials per site (rows of got)"""
+    atol = 0.0
+    if parts is not None:
+        atol = np.asarray(parts, np.float64) * orc_mod.FIX_HALF_STEP
+        if np.ndim(got) == 2:
+            atol = atol[:, None]
+    return Rel(got, ref, ab, atol)
+
+
+def _same_bits(got, ref):
+    """float32 arrays equal bit for bit (the oracle forms the library's exact per-tile sums)"""
+    a = np.ascontiguousarray(got, np.float32)
+    b = np.ascontiguousarray(ref, np.float32)
+    bad = int(np.count_nonzero(a.view(np.uint32) != b.view(np.uint32)))
+    assert a.shape == b.shape and bad == 0, f"{bad} of {a.size} values differ"
 
 
 @pytest.mark.parametrize("removal", [False, True])
@@ -208,9 +223,12 @@ def test_gradients_parity(gpu, orc, w, h, n, passes, removal):
              gpu.put(sites), iss, n, rd, int(removal), 0.0, w, h, w, h)
     acc, ab, racc, rab = orc.gradients(c0, c1, target, mask, sites, iss, w, h, compute_removal=removal)
     got = np.stack([gpu.get(g) for g in grads], 1)
-    assert _rel_ok(got, acc, ab) <= 1.0
+    gf, rf, parts = orc.gradients_fix(c0, c1, target, mask, sites, iss, w, h, compute_removal=removal)
+    assert _rel_ok(got, acc, ab, parts) <= 1.0
+    _same_bits(got, gf)
     if removal:
-        assert _rel_ok(gpu.get(rd), racc, rab) <= 1.0
+        assert _rel_ok(gpu.get(rd), racc, rab, parts) <= 1.0
+        _same_bits(gpu.get(rd), rf)
 
 
 @pytest.mark.parametrize("removal", [False, True])
@@ -232,9 +250,12 @@ def test_gradients_aos_parity(gpu, orc, w, h, n, passes, removal):
     acc, ab, rdr, rda = orc.gradients(c0, c1, target, mask, sites, iss, w, h, compute_removal=removal)
     got = gpu.get(lines)
     assert np.all(got[:, 10:] == 0)
-    assert Rel(got[:, :10], acc, ab) <= 1.0
+    gf, rf, parts = orc.gradients_fix(c0, c1, target, mask, sites, iss, w, h, compute_removal=removal)
+    assert _rel_ok(got[:, :10], acc, ab, parts) <= 1.0
+    _same_bits(got[:, :10], gf)
     if removal:
-        assert Rel(gpu.get(rd), rdr, rda) <= 1.0
+        assert _rel_ok(gpu.get(rd), rdr, rda, parts) <= 1.0
+        _same_bits(gpu.get(rd), rf)
 
 
 def test_render_backward_parity(gpu, orc):
@@ -247,7 +268,9 @@ def test_render_backward_parity(gpu, orc):
     g0, g1 = _fields(gpu, c0, c1)
     gpu.call("sadh_render_backward", g0, g1, gpu.put(go), out, gpu.put(sites), iss, n, w, h, w, h)
     acc, ab = orc.render_backward(c0, c1, go, sites, iss, w, h)
-    assert _rel_ok(gpu.get(out), acc, ab) <= 1.0
+    gf, parts = orc.render_backward_fix(c0, c1, go, sites, iss, w, h)
+    assert _rel_ok(gpu.get(out), acc, ab, parts) <= 1.0
+    _same_bits(gpu.get(out), gf)
 
 
 def test_site_stats_parity(gpu, orc):
@@ -265,6 +288,7 @@ def test_site_stats_parity(gpu, orc):
     got = np.stack([gpu.get(x) for x in st], 1)
     err = np.abs(got - ref) / (np.abs(ref) + 1e-3 * np.abs(ref).max(0) + 1e-20)
     assert err.max() <= 1e-4
+    _same_bits(got, orc.site_stats_fix(c0, c1, target, mask, sites, iss, w, h)[0])
 
 
 # ------------------------------------------------------------- site kernels
@@ -407,6 +431,7 @@ def test_metrics(gpu, orc):
     gpu.call("sadh_compute_psnr", gpu.put(a), gpu.put(b), gpu.put(m), acc, w, h)
     ref = orc.sse(a, b, m)
     assert abs(float(gpu.get(acc)[0]) - ref) <= 1e-5 * ref
+    _same_bits(gpu.get(acc)[:1], np.array([orc.sse_fix(a, b, m)], np.float32))
     sites = random_sites(w, h, 999, 22, inactive_frac=0.3)
     cnt = gpu.zeros((1,), np.uint32)
     gpu.call("sadh_count_active_sites", gpu.put(sites), cnt, 999)
