@@ -136,6 +136,24 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1,
                                  int height, int cand_width, int cand_height,
                                  sadh_stream_t stream);
 
+/* sadh_compute_gradients_tiled without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments, results bit-identical
+ * to sadh_compute_gradients_tiled.  The fixed-point lines live in a device
+ * array of the library, shared by every call on one device: at most 131,072
+ * sites (larger site_count returns SADH_ERR_CAPACITY); calls that overlap on
+ * one device race. */
+int sadh_compute_gradients_tiled_noalloc(const uint32_t* cand0, const uint32_t* cand1,
+                                         const float* target, const float* rendered, const float* mask,
+                                         float* grad_pos_x, float* grad_pos_y, float* grad_log_tau,
+                                         float* grad_radius, float* grad_color_r, float* grad_color_g,
+                                         float* grad_color_b, float* grad_dir_x, float* grad_dir_y,
+                                         float* grad_log_aniso, const SadSite* sites, float inv_scale_sq,
+                                         uint32_t site_count, float* removal_delta,
+                                         uint32_t compute_removal, float ssim_weight, int width,
+                                         int height, int cand_width, int cand_height,
+                                         sadh_stream_t stream);
+
 /* The tiled gradient of sadh_compute_gradients_tiled accumulated into AoS
  * lines: grad_lines is [site_count][16] floats, values 0-9 of site i in Site
  * field order (pos.x, pos.y, log_tau, radius, r, g, b, dir.x, dir.y,

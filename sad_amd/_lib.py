@@ -104,6 +104,8 @@ SIGNATURES = {
     "sadh_hilbert_pairs": [P, P, U32, U32, I32, I32, I32, P],
     "sadh_hilbert_order": [P, P, P, U32, P],
     "sadh_compute_gradients_tiled": [P, P, P, P, P] + [P] * 10 + [P, F32, U32, P, U32, F32, I32, I32, I32, I32, P],
+    "sadh_compute_gradients_tiled_noalloc": [P, P, P, P, P] + [P] * 10 + [P, F32, U32, P, U32, F32, I32, I32, I32,
+                                                                         I32, P],
     "sadh_compute_gradients_tiled_aos": [P, P, P, P, P, P, F32, U32, P, U32, I32, I32, I32, I32, P],
     "sadh_tau_diffuse": [P, P, P, P, P, P, U32, F32, I32, I32, I32, P],
     "sadh_tau_diffuse_adam": [P, P, P, P] + [P] * 10 + [P, F32, I32, I32, I32] + [F32] * 9 + [U32, U32, I32, I32, P],

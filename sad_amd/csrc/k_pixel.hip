@@ -51,6 +51,8 @@
  * sad_grad_terms, the function the oracle evaluates (its square roots from the
  * reproducible sad_rsqrt_nr), so per-pair values agree bit for bit as well.
  */
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "fit_common.h"
@@ -951,6 +953,45 @@ int launch_site_stats_fix(const uint32_t* cand0, const uint32_t* cand1, const fl
 }
 
 
+/* ------------------------------------------ the allocation-free gradient --
+ * sadh_compute_gradients_tiled_noalloc runs the kernels of
+ * sadh_compute_gradients_tiled with their fixed-point lines in `scratch`, a
+ * device array of this library instead of the per-stream workspace, so it
+ * calls no allocator (for callers under graph capture or in loops that must
+ * not allocate).  Every call on a device shares that device's array: calls
+ * that overlap on one device race.  The array is cleared by a kernel at the
+ * start of each call. */
+constexpr uint32_t NOALLOC_MAX_SITES = 131072u;   /* the reference's default site budget is 128,000 */
+constexpr size_t NOALLOC_WORDS = (size_t)NOALLOC_MAX_SITES * (FIX_GRAD_STRIDE + 2u);   /* 27.3 MB */
+
+static __device__ __align__(256) long long scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+long long* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+/* the device address of `scratch` on the current device, looked up once per device */
+static int noalloc_scratch(long long** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail((int)e, std::string("gradients_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(SADH_ERR_UNSUPPORTED, "gradients_noalloc: device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(e != hipSuccess ? (int)e : SADH_ERR_NO_DEVICE,
+                        std::string("gradients_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<long long*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SADH_OK;
+}
+
 }  // namespace sadh
 
 using namespace sadh;
@@ -1030,6 +1071,52 @@ int sadh_compute_gradients_tiled(const uint32_t* cand0, const uint32_t* cand1, c
     return launch_gradients(cand0, cand1, target, mask, g, sites, inv_scale_sq, site_count, removal_delta,
                             compute_removal != 0, width, height, cand_width, cand_height, to_hip(stream), rendered,
                             ssim_weight > 0.0f ? ssim_weight : 0.0f);
+}
+
+int sadh_compute_gradients_tiled_noalloc(const uint32_t* cand0, const uint32_t* cand1, const float* target,
+                                         const float* rendered, const float* mask, float* grad_pos_x,
+                                         float* grad_pos_y, float* grad_log_tau, float* grad_radius,
+                                         float* grad_color_r, float* grad_color_g, float* grad_color_b,
+                                         float* grad_dir_x, float* grad_dir_y, float* grad_log_aniso,
+                                         const SadSite* sites, float inv_scale_sq, uint32_t site_count,
+                                         float* removal_delta, uint32_t compute_removal, float ssim_weight,
+                                         int width, int height, int cand_width, int cand_height,
+                                         sadh_stream_t stream)
+{
+    SADH_REQUIRE(cand0 && cand1 && target && mask && sites, "compute_gradients_tiled_noalloc: null pointer");
+    SADH_REQUIRE(!(ssim_weight > 0.0f) || rendered, "compute_gradients_tiled_noalloc: the SSIM term needs `rendered`");
+    SADH_REQUIRE(grad_pos_x && grad_pos_y && grad_log_tau && grad_radius && grad_color_r && grad_color_g &&
+                     grad_color_b && grad_dir_x && grad_dir_y && grad_log_aniso,
+                 "compute_gradients_tiled_noalloc: null gradient array");
+    SADH_REQUIRE(!compute_removal || removal_delta, "compute_gradients_tiled_noalloc: removal_delta missing");
+    SADH_REQUIRE(width > 0 && height > 0 && cand_width > 0 && cand_height > 0,
+                 "compute_gradients_tiled_noalloc: bad sizes");
+    if (site_count > NOALLOC_MAX_SITES)
+        return fail(SADH_ERR_CAPACITY, "compute_gradients_tiled_noalloc: " + std::to_string(site_count) +
+                                           " sites exceed the scratch of " + std::to_string(NOALLOC_MAX_SITES));
+    long long* lines = nullptr;
+    SAD_TRY(noalloc_scratch(&lines));
+    const hipStream_t S = to_hip(stream);
+    long long* rlines = lines + (size_t)site_count * FIX_GRAD_STRIDE;
+    const size_t words = (size_t)site_count * (FIX_GRAD_STRIDE + (compute_removal ? 2u : 0u));
+    SAD_TRY(launch_clear_fix(lines, words, S));
+    PixelArgs A = {};
+    A.cand0 = cand0; A.cand1 = cand1; A.sites = sites; A.target = target; A.mask = mask;
+    A.rendered = rendered; A.ssim_weight = ssim_weight > 0.0f ? ssim_weight : 0.0f;
+    A.iss = inv_scale_sq; A.n = site_count;
+    A.width = width; A.height = height; A.cw = cand_width; A.ch = cand_height;
+    A.gfix = lines;
+    A.rfix = compute_removal ? rlines : nullptr;
+    if (compute_removal) SAD_TRY((launch_pixel<MODE_GRAD, true, DST_FIX>(A, S, "compute_gradients_tiled_noalloc")));
+    else SAD_TRY((launch_pixel<MODE_GRAD, false, DST_FIX>(A, S, "compute_gradients_tiled_noalloc")));
+    float* const g[10] = {grad_pos_x, grad_pos_y, grad_log_tau, grad_radius, grad_color_r,
+                          grad_color_g, grad_color_b, grad_dir_x, grad_dir_y, grad_log_aniso};
+    FixOut F = {};
+    for (int j = 0; j < 10; ++j) F.g[j] = g[j];
+    F.gstride = 1u; F.nv = 10; F.lstride = FIX_GRAD_STRIDE; F.assign = 0;
+    F.removal = compute_removal ? removal_delta : nullptr;
+    F.lines = lines; F.rlines = rlines; F.n = site_count;
+    return launch_fix_finalize(F, S);
 }
 
 int sadh_compute_gradients_tiled_aos(const uint32_t* cand0, const uint32_t* cand1, const float* target,
