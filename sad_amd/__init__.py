@@ -3,7 +3,7 @@
 libsad_hip.so exports the C ABI of include/sad_hip.h; this package binds it
 with ctypes (sad_amd._lib), drives the native fit/render loops
 (sad_amd.trainer), registers the torch.ops.sad operator set on HIP tensors
-(sad_amd.ops) and provides the autograd render (sad_amd.modules).
+(sad_amd.ops) and provides the differentiable render (sad_amd.differentiable).
 """
 from . import _lib
 from ._lib import SadHipError, load

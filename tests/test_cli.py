@@ -170,18 +170,17 @@ def test_cli_fit_and_render(tmp_path, capsys):
     assert png.exists() and txt.exists()
     sites, w, h = sites_io.load_sites(str(txt))
     assert (w, h) == (96, 64) and np.all(sites[:, 0] >= 0)
-    # the same fit through the library (gradient sums are fp32 atomics, so the
-    # two runs agree to rounding): same schedule, PSNR of the written PNG within
-    # 0.05 dB of both runs' final PSNR
+    # the same fit through the library ends with the same bits (fits repeat exactly; the sites file's %.9g
+    # round-trips float32), and the 8-bit PNG's PSNR is within 0.05 dB of the final PSNR
     tq = sites_io.load_image(str(img))
     final = float(log.split("Final PSNR: ")[1].split()[0])
     o = cli.parse_args([str(img), "--iters", "60", "--sites", "600", "--target-bpp", "1.5", "--log-freq", "20"],
                        config.DEFAULT_CONFIG)["opts"]
     res = trainer.fit(tq, None, o, want_render=True)
     ref = sites_io.active_sites(res["sites"][: res["actual_sites"]])
-    assert abs(sites.shape[0] - ref.shape[0]) <= 0.01 * ref.shape[0]
+    assert sites.shape == ref.shape and np.array_equal(sites.view(np.uint32), ref.view(np.uint32))
     assert abs(trainer.psnr(sites_io.load_image(str(png)), tq) - final) <= 0.05
-    assert abs(res["final_psnr"] - final) <= 0.05
+    assert abs(res["final_psnr"] - final) <= 5e-4
     assert abs(trainer.psnr(res["render"], tq) - res["final_psnr"]) <= 0.01
     # render mode on the written sites file, PSNR against the source image
     rpng = tmp_path / "r.png"

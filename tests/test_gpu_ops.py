@@ -50,21 +50,26 @@ def test_vpt_pass_bit_exact(gpu, orc, sad):
 
 
 def test_autograd_render_matches_oracle_and_fd64(gpu, orc):
-    """SADRenderFunction backward vs the oracle backward (1e-4 relative, with
-    the cancellation allowance) and vs FP64 central differences."""
-    from sad_amd.modules import SADRenderer
+    """The differentiable render's backward vs the oracle backward (bit for bit
+    against its tile-ordered sums; 1e-4 relative, with the cancellation
+    allowance, against its float64 sums) and vs FP64 central differences."""
+    from sad_amd.differentiable import CandidateField, inv_scale_sq, render
     torch = gpu.torch
     w, h = 64, 64
     sites, go = fd_problem(w=w, h=h)
     c0, c1 = converged_field(orc, sites, w, h, passes=10)
     iss = iss_of(w, h)
+    assert inv_scale_sq(w, h) == iss
     ts = gpu.put(sites).requires_grad_(True)
-    out = SADRenderer(iss)(ts, gpu.put(c0), gpu.put(c1), w, h)
+    field = CandidateField(w, h, gpu.put(c0), gpu.put(c1))
+    out = render(ts, field)
     assert np.abs(gpu.get(out) - orc.render(c0, c1, sites, iss, w, h)).max() <= 1e-5
     (out * gpu.put(go)).sum().backward()
     got = gpu.get(ts.grad)
     acc, ab = orc.render_backward(c0, c1, go, sites, iss, w, h)
-    assert Rel(got, acc, ab) <= 1.0
+    exact, parts = orc.render_backward_fix(c0, c1, go, sites, iss, w, h)
+    assert np.array_equal(got.view(np.uint32), exact.view(np.uint32))
+    assert Rel(got, acc, ab, parts[:, None] * orc.FIX_HALF_STEP) <= 1.0
     R = go.astype(np.float64)
 
     def loss(s):
@@ -135,3 +140,23 @@ def test_ops_reject_bad_shapes(gpu, sad):
         sad.render_sad(c, c, torch.zeros((2, 9), device=gpu.dev), 0.1, 4, 4)
     with pytest.raises(RuntimeError, match="cand0 size must match"):
         sad.render_sad(c, c, torch.zeros((2, 10), device=gpu.dev), 0.1, 5, 4)
+
+
+def test_candidate_field_seeded_and_refined(gpu, orc):
+    """CandidateField.seeded is init + JFA seed, refine() runs VPT passes on the
+    jump schedule; both equal the oracle's fields bit for bit."""
+    from sad_amd.differentiable import CandidateField, jump_schedule
+    w, h, n = 80, 64, 500
+    sites = random_sites(w, h, n, 23)
+    ts = gpu.put(sites)
+    f = CandidateField.seeded(ts, w, h, seed=3)
+    r0, r1 = orc.init_candidates(n, 3, False, w, h)
+    r0 = orc.jfa_seed(r0, sites, w, h)
+    assert np.array_equal(_u32(gpu, f.cand0), r0) and np.array_equal(_u32(gpu, f.cand1), r1)
+    assert jump_schedule(w, h, 8) == [orc.jump_step(i, w, h) for i in range(8)]
+    f.refine(ts, passes=5)
+    pk = orc.pack_sites(sites)
+    for jump in jump_schedule(w, h, 5):
+        r0, r1 = orc.update_candidates(r0, r1, pk, jump << 16, iss_of(w, h), w, h, inject_count=4)
+    assert f.passes_done == 5
+    assert np.array_equal(_u32(gpu, f.cand0), r0) and np.array_equal(_u32(gpu, f.cand1), r1)
