@@ -28,7 +28,9 @@ Other workloads (not the headline line):
                          (sad_amd.kernel_bench) with the CPU oracle on the same
                          inputs; the line carries every kernel's row.
   --workload batch64     64 images (about 60 MP) fitted as one batch for 500
-                         iterations, and one at a time on the same inputs.
+                         iterations, steady ticks replayed as captured
+                         hipGraphs (--graph-ticks 0: launched on the stream),
+                         and one at a time on the same inputs.
   --workload serve       a Poisson stream of fit and render requests replayed
                          against the continuous-batching server: throughput
                          and p50/p99 latency per request kind.
@@ -173,22 +175,27 @@ def batch_workload(args) -> int:
     targets, sites = synth.batch_workload()
     iters = 500
     opts = config.fit_options(iterations=iters, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
-                              prune_percentile=0.0, log_freq=10**9, profile_gradients=1)
+                              prune_percentile=0.0, log_freq=10**9)
+    # the timed batches record no per-launch events (a profiled tick is never graph-replayed); one more
+    # batch with profile_gradients times the batched gradient launches for the roofline
+    bopts = dict(opts, graph_ticks=args.graph_ticks)
     mp = sum(t.shape[0] * t.shape[1] for t in targets) / 1e6
     for _ in range(args.warmup):
-        trainer.fit_batch(targets[:4], sites[:4], opts)
+        trainer.fit_batch(targets[:4], sites[:4], bopts)
     torch.cuda.synchronize()
-    runs = [trainer.fit_batch(targets, sites, opts) for _ in range(args.steps)]
+    runs = [trainer.fit_batch(targets, sites, bopts) for _ in range(args.steps)]
+    prof = trainer.fit_batch(targets, sites, dict(opts, profile_gradients=1))
     loop_s = sum(r["train_seconds"] for r in runs)
     value = mp * iters * args.steps / loop_s
     last = runs[-1]
-    single_s, dpsnr = 0.0, 0.0
+    single_s, dpsnr, all_same = 0.0, 0.0, True
     for i, (t, s0) in enumerate(zip(targets, sites)):
         r = trainer.fit(t, s0, opts)
         single_s += r["train_seconds"]
         dpsnr = max(dpsnr, abs(r["final_psnr"] - last["jobs"][i]["final_psnr"]))
+        all_same = all_same and np.array_equal(r["sites"].view(np.uint32), last["jobs"][i]["sites"].view(np.uint32))
     single = mp * iters / single_s
-    gms, gb = last["grad_kernel_ms"], last["grad_bytes"]
+    gms, gb = prof["grad_kernel_ms"], prof["grad_bytes"]
     achieved = gb / (gms * 1e-3) / 1e9 if gms > 0 else 0.0
     cpu = None
     if not args.no_cpu:
@@ -208,14 +215,16 @@ def batch_workload(args) -> int:
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * loop_s / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch64", "images": len(targets), "megapixels": round(mp, 3), "iterations": iters,
-                   "sites": int(sum(len(s0) for s0 in sites)),
+                   "sites": int(sum(len(s0) for s0 in sites)), "graph_ticks": last["graph_ticks"],
+                   "graph_captures": last["graph_captures"],
+                   "jobs_bit_identical_to_single": bool(all_same),
                    "mean_final_psnr_db": round(float(np.mean([j["final_psnr"] for j in last["jobs"]])), 3)},
         "single_image_path": {"value": round(single, 3), "unit": "MP-iters/s", "seconds": round(single_s, 3),
                               "max_final_psnr_diff_db": round(dpsnr, 4)},
         "speedup_vs_single_image_path": round(value / single, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_pixel_grad_batch", "avg_launch_us": round(1e3 * gms / max(1, last["grad_launches"]), 3)},
+                     "kernel": "k_pixel_grad_batch", "avg_launch_us": round(1e3 * gms / max(1, prof["grad_launches"]), 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -264,6 +273,8 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate per second")
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-ticks", type=int, default=1, choices=[0, 1],
+                    help="batch64: replay steady ticks as captured hipGraphs (1) or launch them on the stream (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fit path runs on one device)",
