@@ -119,15 +119,20 @@ def run(size: int = 4096, n_sites: int = 1 << 20, reps: int = 20, seed: int = 3,
     pk = torch.empty((n, 8), dtype=torch.float16, device=dev)
     rows.append(timed(lambda: call("sadh_pack_candidate_sites", gs, pk, n), "k_pack_sites", 56 * n, f"N={n}"))
 
-    def vpt(c0, c1, o0, o1, idx):
+    def vpt(c0, c1, o0, o1, idx, inject=4, hil_probes=1):
         j = max(1, (1 << (bits - 1)) >> min(idx, bits - 1))
-        call("sadh_update_candidates", c0, c1, o0, o1, pk, n, (j << 16) | (idx & 0xFFFF), iss, 0, 64.0, 0, 4, order,
-             pos, 1, 32, 1, w, h, w, h)
+        call("sadh_update_candidates", c0, c1, o0, o1, pk, n, (j << 16) | (idx & 0xFFFF), iss, 0, 64.0, 0, inject,
+             order, pos, hil_probes, 32, 1, w, h, w, h)
 
     for idx in range(bits + 4):
         vpt(a0, a1, b0, b1, idx)
         a0, b0, a1, b1 = b0, a0, b1, a1
     rows.append(timed(lambda: vpt(a0, a1, b0, b1, 100), "k_vpt (jump 1, Hilbert 1/32, inject 4)",
+                      64 * P + 24 * n, f"P={P}, N={n}"))
+    # the same pass without its random-id sources: Hilbert neighbours, then also the injected ids
+    rows.append(timed(lambda: vpt(a0, a1, b0, b1, 100, 4, 0), "k_vpt (jump 1, no Hilbert, inject 4)",
+                      64 * P + 24 * n, f"P={P}, N={n}"))
+    rows.append(timed(lambda: vpt(a0, a1, b0, b1, 100, 0, 0), "k_vpt (jump 1, no Hilbert, no injection)",
                       64 * P + 24 * n, f"P={P}, N={n}"))
     # ---- render, gradient, tau, Adam
     out = torch.empty((h, w, 3), device=dev)
