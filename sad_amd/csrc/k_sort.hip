@@ -2,15 +2,19 @@
  * k_sort.hip - stable LSD radix sort of (key, value) uint32 pairs by key,
  * 8-bit digits (launchRadixSortUInt2, radix_sort.cu:279-336).
  *
- * Per digit pass, three launches:
+ * Per digit pass, three launches, each spread over the whole device:
  *   1. per-tile digit histogram (tile = 1024 pairs = 4 chunks of 256),
  *      written digit-major: hist[digit * tiles + tile];
- *   2. one-workgroup exclusive scan of hist (1024 threads), which turns every
- *      entry into the global output offset of (digit, tile);
- *   3. scatter: each 256-pair chunk is ranked stably with wave64 ballots (the
- *      8 digit bits are matched with 8 ballots, rank = popcount of the matching
- *      lower lanes), waves are ordered through LDS counts, and a per-digit
- *      running offset carries the order from chunk to chunk.
+ *   2. one wave per digit turns its row of hist into an inclusive scan over
+ *      the tiles; the row's last entry is then the digit's total;
+ *   3. scatter: each block forms the 256 digit bases (an exclusive scan of the
+ *      row totals) and adds its tile's exclusive row prefix, then ranks each
+ *      256-pair chunk stably with wave64 ballots (the 8 digit bits are matched
+ *      with 8 ballots, rank = popcount of the matching lower lanes), orders the
+ *      waves through LDS counts, and carries a per-digit running offset from
+ *      chunk to chunk.
+ * Only the caller's hist words are used (256 per tile), so the reference's
+ * buffer sizes hold (block_sums is not needed).
  * The reference ranks each element with an O(256) loop over the chunk
  * (radix_sort.cu:244-253); the ballot match is O(8) per element.
  * Output order is exactly that of a stable sort by key, so it is bit-exact
@@ -39,39 +43,33 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint2* __restrict__ in,
     hist[t * tiles + blockIdx.x] = h[t];
 }
 
-/* Exclusive scan of len words (len a multiple of 4) in place by one
- * 1024-thread workgroup, in chunks of 4096: each thread loads 4 consecutive
- * words as one uint4 (coalesced), scans them, and a wave scan plus a scan of
- * the 16 wave totals gives the chunk's offsets; a carry links the chunks. */
-__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ data, uint32_t len)
+/* Inclusive scan of each digit's row hist[d * tiles .. + tiles) in place, one
+ * 64-lane wave per digit: chunks of 256 words, 4 consecutive words per lane,
+ * a wave scan of the lane totals, and a carry from chunk to chunk. */
+__global__ __launch_bounds__(64) void k_sort_rowscan(uint32_t* __restrict__ hist, uint32_t tiles)
 {
-    __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t* row = hist + (size_t)blockIdx.x * tiles;
+    const uint32_t lane = threadIdx.x;
     uint32_t carry = 0u;
-    for (uint32_t base = 0; base < len; base += 4096u) {
-        const uint32_t i0 = base + 4u * t;
-        uint4 x = make_uint4(0u, 0u, 0u, 0u);
-        if (i0 < len) x = *reinterpret_cast<const uint4*>(data + i0);
-        const uint32_t s1 = x.x + x.y, s2 = s1 + x.z, tot = s2 + x.w;
-        uint32_t v = tot;
+    for (uint32_t base = 0; base < tiles; base += 256u) {
+        const uint32_t i0 = base + 4u * lane;
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4u; ++k) x[k] = (i0 + k < tiles) ? row[i0 + k] : 0u;
+        x[1] += x[0];
+        x[2] += x[1];
+        x[3] += x[2];
+        uint32_t v = x[3];
 #pragma unroll
         for (uint32_t off = 1; off < 64u; off <<= 1) {
-            uint32_t o = (uint32_t)__shfl_up((int)v, off);
+            const uint32_t o = (uint32_t)__shfl_up((int)v, off);
             if (lane >= off) v += o;
         }
-        if (lane == 63u) wsum[w] = v;
-        __syncthreads();
-        uint32_t before = 0u, all = 0u;
+        const uint32_t before = carry + v - x[3];
 #pragma unroll
-        for (uint32_t j = 0; j < 16u; ++j) {
-            const uint32_t ws = wsum[j];
-            before += (j < w) ? ws : 0u;
-            all += ws;
-        }
-        const uint32_t ex = carry + before + v - tot;
-        if (i0 < len) *reinterpret_cast<uint4*>(data + i0) = make_uint4(ex, ex + x.x, ex + s1, ex + s2);
-        carry += all;
-        __syncthreads();
+        for (uint32_t k = 0; k < 4u; ++k)
+            if (i0 + k < tiles) row[i0 + k] = before + x[k];
+        carry += (uint32_t)__shfl((int)v, 63);
     }
 }
 
@@ -81,8 +79,26 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint2* __restrict__ 
 {
     __shared__ uint32_t running[256];
     __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t wtot[4];
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    running[t] = offs[t * tiles + blockIdx.x];
+    {
+        /* offs holds each digit's inclusive scan over the tiles: its last entry is the digit's total, and the
+         * entry before this tile is the pairs of this digit in earlier tiles */
+        const uint32_t* row = offs + (size_t)t * tiles;
+        const uint32_t tot = row[tiles - 1u];
+        const uint32_t prev = blockIdx.x ? row[blockIdx.x - 1u] : 0u;
+        uint32_t v = tot;
+#pragma unroll
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)v, off);
+            if (lane >= off) v += o;
+        }
+        if (lane == 63u) wtot[w] = v;
+        __syncthreads();
+        uint32_t base = v - tot;
+        for (uint32_t j = 0; j < w; ++j) base += wtot[j];
+        running[t] = base + prev;
+    }
     const unsigned long long lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
     const uint32_t base = blockIdx.x * SORT_TILE;
     for (uint32_t c = 0; c < 4u; ++c) {
@@ -125,7 +141,7 @@ int launch_radix_sort(SadPair* data, SadPair* scratch, uint32_t* hist, uint32_t 
     for (int p = 0; p < passes; ++p) {
         uint32_t shift = 8u * (uint32_t)p;
         hipLaunchKernelGGL(k_sort_hist, dim3(tiles), dim3(256), 0, stream, a, hist, n, tiles, shift);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, stream, hist, 256u * tiles);
+        hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(64), 0, stream, hist, tiles);
         hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(256), 0, stream, a, b, hist, n, tiles, shift);
         int rc = launch_status("radix_sort_pairs");
         if (rc) return rc;
