@@ -325,7 +325,9 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         uint4 a = none, b = none;
         float mk = 1.0f;
         if (inb) {
-            int cell = sad_cand_index(x, y, A.width, A.height, A.cw, A.ch);
+            /* a full-resolution field (the usual case) needs none of the index scaling's divisions */
+            const int cell = (A.cw == A.width && A.ch == A.height) ? y * A.width + x
+                                                                     : sad_cand_index(x, y, A.width, A.height, A.cw, A.ch);
             a = reinterpret_cast<const uint4*>(A.cand0)[cell];
             b = reinterpret_cast<const uint4*>(A.cand1)[cell];
             if constexpr (MODE == MODE_GRAD || MODE == MODE_STATS) mk = A.mask[pix];
@@ -368,7 +370,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         valid[k] = s != SLOT_INVALID && t.p0.x >= 0.0f;
         float proj, perp2;
         float d2s = d2_safe(t, ux - t.p0.x, uy - t.p0.y, t.p1.w, A.iss, &proj, &perp2);
-        logit[k] = valid[k] ? -(t.p0.z * (sqrtf(d2s) - t.p0.w)) : -INFINITY;
+        logit[k] = valid[k] ? -(t.p0.z * (sad_sqrt_pos(d2s) - t.p0.w)) : -INFINITY;
         mx = fmaxf(mx, logit[k]);
     }
     bool has = mx > -INFINITY;
@@ -432,8 +434,10 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             }
         }
     }
-    L.px[tid] = make_float4(pr, pgr, pb, err);
-    L.pg[tid] = make_float4(gr, gg, gb, 0.0f);
+    /* the statistics read the pixel's error from px.w; the gradients read the pixel's coordinates there and in
+     * pg.w, which saves decoding them from the entry */
+    L.px[tid] = make_float4(pr, pgr, pb, MODE == MODE_STATS ? err : ux);
+    L.pg[tid] = make_float4(gr, gg, gb, uy);
     if (REMOVAL) L.pt[tid] = make_float4(tr, tg, tb, 0.0f);
     /* the pairs that are summed keep their slot; a valid id without a slot waits for a later round */
     bool pending = false;
@@ -549,9 +553,10 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         for (uint32_t e = beg + part; e < end; e += CLANES) {
             const uint32_t ent = L.list[e];
             const uint32_t p = (ent & 0x7FFFu) >> 3, k = ent & 7u;
-            const float ex = (float)(tx0 + (int)(p & 15u)), ey = (float)(ty0 + (int)(p >> 4));
             const float4 pp = L.px[p];
             const float4 gq = L.pg[p];
+            const float ex = MODE == MODE_STATS ? (float)(tx0 + (int)(p & 15u)) : pp.w;
+            const float ey = MODE == MODE_STATS ? (float)(ty0 + (int)(p >> 4)) : gq.w;
             if (MODE == MODE_STATS || (MODE == MODE_GRAD && REMOVAL)) {
                 if (ent & ENTRY_FIRST) {
                     float wt = 0.0f;
@@ -579,13 +584,12 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
                 for (int j = 0; j < 10; ++j) acc[j] += c[j];
             }
         }
-        /* every lane of the group ends with the same total; lane `part` stores the values j = part mod CLANES */
+        /* every lane of the group ends with the same totals; the group's first lane stores them */
 #pragma unroll
         for (int j = 0; j < NV; ++j) acc[j] = chunk_sum(acc[j]);
-        if (live) {
+        if (live && part == 0u) {
 #pragma unroll
-            for (int j = 0; j < NV; ++j)
-                if (((uint32_t)j & (CLANES - 1u)) == part) L.part[q][j] = acc[j];
+            for (int j = 0; j < NV; ++j) L.part[q][j] = acc[j];
         }
     }
     __syncthreads();

@@ -107,7 +107,9 @@ __device__ __forceinline__ float h2f(uint32_t bits16)
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v)
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    /* every lane of the patterns used here reads an existing lane, so bound_ctrl changes no value; with it set
+     * the compiler folds the move into the consuming add or max (v_add_f32_dpp) */
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float group8_sum(float v)
 {

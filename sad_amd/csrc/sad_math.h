@@ -53,16 +53,18 @@ SAD_HD float sad_pow2i(int k) { return sad_u2f((uint32_t)(k + 127) << 23); }
 
 SAD_HD float sad_expf(float x)
 {
-    if (!(x == x)) return x + x;                 /* NaN */
-    if (x > 0x1.62e430p+6f) return INFINITY;
-    if (x < -103.97208404541015625f) return 0.0f;
+    /* The polynomial path runs on x clamped to [-104, 89] (NaN to -104), so its
+     * integer conversions stay in range; the special cases then select their
+     * results.  Inside [-103.97.., 88.72..] the clamp changes nothing, and there
+     * are no branches for a GPU to diverge on. */
+    const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
     const float L2E = 0x1.715476p+0f;   /* log2(e) */
     const float LN2_HI = 0x1.62e400p-1f;   /* ln2 head, 16 significant bits */
     const float LN2_LO = 0x1.7f7d1cp-20f;  /* ln2 - head */
     const float SHIFTER = 12582912.0f;                     /* 1.5 * 2^23 */
-    float t = x * L2E;
+    float t = xc * L2E;
     float kf = (t + SHIFTER) - SHIFTER;          /* round-to-nearest-even of t */
-    float r = fmaf(-kf, LN2_HI, x);
+    float r = fmaf(-kf, LN2_HI, xc);
     r = fmaf(-kf, LN2_LO, r);
     float p = 0x1.a01a02p-13f;   /* 1/5040 */
     p = fmaf(p, r, 0x1.6c16c2p-10f);  /* 1/720 */
@@ -75,7 +77,29 @@ SAD_HD float sad_expf(float x)
     int k = (int)kf;
     int k1 = k / 2;
     int k2 = k - k1;
-    return (p * sad_pow2i(k1)) * sad_pow2i(k2);
+    float e = (p * sad_pow2i(k1)) * sad_pow2i(k2);
+    e = (x < -103.97208404541015625f) ? 0.0f : e;
+    e = (x > 0x1.62e430p+6f) ? INFINITY : e;
+    return (x == x) ? e : x + x;               /* NaN */
+}
+
+/* sqrtf of x >= 2^-96 (or +inf), correctly rounded.  On the GPU: the hardware
+ * square root (within 1 ulp) and one correction step that compares the
+ * residuals x - s'*s of the neighbouring floats s' = s -/+ 1 ulp with zero,
+ * which is the sequence the compiler emits for sqrtf, less the scaling of tiny
+ * inputs and the zero/infinity fix-up that this range does not need.  The
+ * soft distance's argument is at least 1e-8.  On a CPU: sqrtf. */
+SAD_HD float sad_sqrt_pos(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = sad_u2f(sad_f2u(s) - 1u);
+    const float su = sad_u2f(sad_f2u(s) + 1u);
+    const float t = (fmaf(-sd, s, x) <= 0.0f) ? sd : s;
+    return (fmaf(-su, s, x) > 0.0f) ? su : t;
+#else
+    return sqrtf(x);
+#endif
 }
 
 /* sad_logf (x > 0): x = m * 2^e with m in [sqrt(1/2), sqrt(2)), then
@@ -272,7 +296,7 @@ SAD_HD float sad_dmix2_pre(float px, float py, float dirx, float diry,
     float perp2 = fmaxf(diff2 - proj2, 0.0f);
     float d2a = fmaf(l1, proj2, l2 * perp2);
     float d2s = fmaxf(d2a * inv_scale_sq, 1e-8f);
-    return sqrtf(d2s) - r_norm;
+    return sad_sqrt_pos(d2s) - r_norm;
 }
 
 SAD_HD float sad_dmix2(float px, float py, float dirx, float diry, float log_aniso,
