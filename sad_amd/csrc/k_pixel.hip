@@ -379,7 +379,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     float sum = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        w[k] = (valid[k] && has) ? sad_expf(logit[k] - mx) : 0.0f;
+        w[k] = (valid[k] && has) ? sad_expf_nonpos(logit[k] - mx) : 0.0f;   /* logit - max <= 0 */
         sum += w[k];
     }
     const float inv = 1.0f / fmaxf(sum, 1e-8f);
@@ -524,9 +524,12 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     for (int k = 0; k < 8; ++k) {
         const uint8_t s = sl[k];
         if (s != SLOT8_NONE) {
-            bool first = true;
+            /* the first rank of a site in the pixel: only the statistics and removal deltas read it */
+            bool first = MODE == MODE_STATS || REMOVAL;
+            if (MODE == MODE_STATS || REMOVAL) {
 #pragma unroll
-            for (int j = 0; j < k; ++j) first = first && (sl[j] != s);
+                for (int j = 0; j < k; ++j) first = first && (sl[j] != s);
+            }
             const uint32_t at = L.start[s] + ((L.wcount[wv >> 1][s] >> wshift) & 0xFFFFu) + rank[k];
             L.list[at] = (uint16_t)((tid << 3) | (uint32_t)k | (first ? ENTRY_FIRST : 0u));
         }
@@ -579,7 +582,8 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             }
             if (MODE != MODE_STATS) {
                 float c[10];
-                candidate_grad(t, ex, ey, L.w[p][k], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss, A.inv_scale, c);
+                candidate_grad(t, ex, ey, (&L.w[0][0])[ent & 0x7FFFu], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss,
+                               A.inv_scale, c);
 #pragma unroll
                 for (int j = 0; j < 10; ++j) acc[j] += c[j];
             }
@@ -607,14 +611,13 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             const uint32_t qa = L.cstart[s], qb = qa + (c + CHUNK - 1u) / CHUNK;
             float v = 0.0f;
             for (uint32_t q = qa; q < qb; ++q) v += L.part[q][j];
-            long long lo, hi;
-            fix_split(v, &lo, &hi);
+            const long long word = fix_word(v, hiw);
             const size_t id = L.key[s];
             long long* dst;
             if (DST == DST_FIXSTATS) dst = A.sfix + id * FIX_STAT_STRIDE + 2u * j;
             else if (REMOVAL && j == 10u) dst = A.rfix + 2u * id;
             else dst = A.gfix + id * FIX_GRAD_STRIDE + 2u * j;
-            fix_atomic_add(dst + hiw, hiw ? hi : lo);
+            fix_atomic_add(dst + hiw, word);
         }
     } else {
         for (uint32_t i = tid; i < n_occ * (uint32_t)NV; i += 256u) {

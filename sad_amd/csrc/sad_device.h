@@ -71,6 +71,16 @@ __device__ __forceinline__ void fix_split(float v, long long* lo, long long* hi)
     *lo = (long long)rintf((v - t) * 0x1p40f);
 }
 
+/* one word of fix_split(v): hi (hiw = 1) or lo.  Both are integral floats below
+ * 2^62 in magnitude, so one conversion of the selected one gives the same word. */
+__device__ __forceinline__ long long fix_word(float v, uint32_t hiw)
+{
+    if (!(v == v)) v = 0.0f;
+    v = fminf(fmaxf(v, -0x1p62f), 0x1p62f);
+    const float t = truncf(v);
+    return (long long)(hiw ? t : rintf((v - t) * 0x1p40f));
+}
+
 __device__ __forceinline__ float fix_value(long long lo, long long hi)
 {
     return (float)((double)hi + (double)lo * 0x1p-40);

@@ -51,13 +51,12 @@ SAD_HD float sad_clampf(float x, float a, float b) { return fmaxf(a, fminf(b, x)
  * the result is bit-identical on every IEEE binary32 target. */
 SAD_HD float sad_pow2i(int k) { return sad_u2f((uint32_t)(k + 127) << 23); }
 
-SAD_HD float sad_expf(float x)
+/* e^xc for xc in [-104, 89] (the caller's clamp): the reduction, the
+ * polynomial and 2^k in two exact power-of-two steps.  k is split as
+ * floor(k/2) + the rest; any split whose first factor keeps p * 2^k1 normal
+ * gives the same, once-rounded product. */
+SAD_HD float sad_exp_core(float xc)
 {
-    /* The polynomial path runs on x clamped to [-104, 89] (NaN to -104), so its
-     * integer conversions stay in range; the special cases then select their
-     * results.  Inside [-103.97.., 88.72..] the clamp changes nothing, and there
-     * are no branches for a GPU to diverge on. */
-    const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
     const float L2E = 0x1.715476p+0f;   /* log2(e) */
     const float LN2_HI = 0x1.62e400p-1f;   /* ln2 head, 16 significant bits */
     const float LN2_LO = 0x1.7f7d1cp-20f;  /* ln2 - head */
@@ -75,12 +74,30 @@ SAD_HD float sad_expf(float x)
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     int k = (int)kf;
-    int k1 = k / 2;
+    int k1 = k >> 1;                             /* arithmetic shift: floor(k / 2), at least -75 */
     int k2 = k - k1;
-    float e = (p * sad_pow2i(k1)) * sad_pow2i(k2);
+    return (p * sad_pow2i(k1)) * sad_pow2i(k2);
+}
+
+SAD_HD float sad_expf(float x)
+{
+    /* The polynomial path runs on x clamped to [-104, 89] (NaN to -104), so its
+     * integer conversions stay in range; the special cases then select their
+     * results.  Inside [-103.97.., 88.72..] the clamp changes nothing, and there
+     * are no branches for a GPU to diverge on. */
+    float e = sad_exp_core(fminf(fmaxf(x, -104.0f), 89.0f));
     e = (x < -103.97208404541015625f) ? 0.0f : e;
     e = (x > 0x1.62e430p+6f) ? INFINITY : e;
     return (x == x) ? e : x + x;               /* NaN */
+}
+
+/* sad_expf for x <= 0 or NaN, with the same bits: the softmax's logit - max,
+ * which never needs the overflow case. */
+SAD_HD float sad_expf_nonpos(float x)
+{
+    float e = sad_exp_core(fmaxf(x, -104.0f));
+    e = (x < -103.97208404541015625f) ? 0.0f : e;
+    return (x == x) ? e : x + x;
 }
 
 /* sqrtf of x >= 2^-96 (or +inf), correctly rounded.  On the GPU: the hardware
