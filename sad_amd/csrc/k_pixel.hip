@@ -383,6 +383,27 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         sum += w[k];
     }
     const float inv = 1.0f / fmaxf(sum, 1e-8f);
+    /* the candidates' colours: the staged ones are read from LDS for every rank first (an index in range for
+     * every slot value), and a pending id's colour replaces its entry in a separate pass, so the compiler keeps
+     * LDS loads here instead of merging the two sources into generic loads */
+    float cr[8], cg[8], cb[8];
+    bool any_pending = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint16_t s = slots[k];
+        const float4 q = L.p2[s < (uint16_t)H ? s : 0u];
+        cr[k] = q.x; cg[k] = q.y; cb[k] = q.z;
+        any_pending = any_pending || s == SLOT_OVERFLOW;
+    }
+    if (any_pending) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (slots[k] == SLOT_OVERFLOW) {
+                const SadSite* sp = A.sites + ids[k];
+                cr[k] = sp->col_r; cg[k] = sp->col_g; cb[k] = sp->col_b;
+            }
+        }
+    }
     float pr = 0.0f, pgr = 0.0f, pb = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -390,7 +411,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         float wc = w[k];
         if (MODE == MODE_RENDER && (isnan(wc) || isinf(wc))) wc = 0.0f;
         if (valid[k]) {
-            float4 c = (slots[k] < (uint16_t)H) ? L.p2[slots[k]] : site_terms(A.sites, ids[k], A.inv_scale).p2;
+            float4 c = make_float4(cr[k], cg[k], cb[k], 0.0f);
             if (MODE == MODE_RENDER && A.cells) {
                 float hc[3];
                 sad_hash_color(ids[k], hc);
