@@ -52,6 +52,33 @@ struct BatchJob {
     uint32_t site_begin;
 };
 
+/* Job j of the table with its pointers rebased on the table's own (kernel
+ * argument) address: the same addresses, accessed with global rather than
+ * generic memory instructions (rebase, sad_device.h). */
+__device__ __forceinline__ BatchJob job_global(const BatchJob* __restrict__ jobs, uint32_t j)
+{
+    BatchJob J = jobs[j];
+    J.target = rebase(jobs, J.target);
+    J.mask = rebase(jobs, J.mask);
+    J.fa0 = rebase(jobs, J.fa0);
+    J.fa1 = rebase(jobs, J.fa1);
+    J.fb0 = rebase(jobs, J.fb0);
+    J.fb1 = rebase(jobs, J.fb1);
+    J.sites = rebase(jobs, J.sites);
+    J.adam = rebase(jobs, J.adam);
+    J.search = rebase(jobs, J.search);
+    J.gacc = rebase(jobs, J.gacc);
+    J.tau_raw = rebase(jobs, J.tau_raw);
+    J.tau_tmp = rebase(jobs, J.tau_tmp);
+    J.tau_tmp2 = rebase(jobs, J.tau_tmp2);
+    J.tau_nbr = rebase(jobs, J.tau_nbr);
+    J.removal = rebase(jobs, J.removal);
+    J.rendered = rebase(jobs, J.rendered);
+    J.h_order = rebase(jobs, J.h_order);
+    J.h_pos = rebase(jobs, J.h_pos);
+    return J;
+}
+
 /* job owning flattened index f, given each job's first index `begin` (ascending) */
 __device__ __forceinline__ uint32_t batch_find(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t f,
                                                uint32_t BatchJob::*begin)

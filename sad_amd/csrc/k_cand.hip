@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void k_vpt_batch(const BatchJob* __restrict__ 
     const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::ctile_begin);
     if (!steps[j].vpt) return;
     const uint32_t jump_index = steps[j].jump_index + st.vpt_pass;
-    const BatchJob& J = jobs[j];
+    const BatchJob J = job_global(jobs, j);
     const uint32_t t = f - J.ctile_begin;
     const int x = (int)(t % J.ctiles_x) * 16 + (int)(threadIdx.x & 15u);
     const int y = (int)(t / J.ctiles_x) * 16 + (int)(threadIdx.x >> 4);
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_search_batch(const BatchJob* __restrict
     const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::site_begin);
     if (!steps[j].vpt) return;
-    const BatchJob& J = jobs[j];
+    const BatchJob J = job_global(jobs, j);
     const uint32_t i = f - J.site_begin;
     if (i >= J.n) return;
     SadSite s = J.sites[i];

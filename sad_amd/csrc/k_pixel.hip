@@ -702,7 +702,7 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     __shared__ PixelLds<MODE_GRAD, REMOVAL, H> L;
     const uint32_t f = xcd_tile(blockIdx.x, total_tiles);
     const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::tile_begin);
-    const BatchJob& J = jobs[j];
+    const BatchJob J = job_global(jobs, j);
     PixelArgs A = {};
     A.cand0 = J.cur ? J.fb0 : J.fa0;
     A.cand1 = J.cur ? J.fb1 : J.fa1;

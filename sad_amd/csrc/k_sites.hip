@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
 {
     const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     const uint32_t jx = batch_find(jobs, njobs, f, &BatchJob::site_begin);
-    const BatchJob& J = jobs[jx];
+    const BatchJob J = job_global(jobs, jx);
     const JobStep js = steps[jx];
     const uint32_t i = f - J.site_begin;
     if (i >= J.n) return;

@@ -71,6 +71,18 @@ __device__ __forceinline__ void fix_split(float v, long long* lo, long long* hi)
     *lo = (long long)rintf((v - t) * 0x1p40f);
 }
 
+/* A pointer loaded from device memory (a job table) is generic to the compiler,
+ * so accesses through it compile to generic (flat) memory instructions.
+ * Rebuilt as an offset from a kernel argument (which the compiler knows to be
+ * global memory), the same address is accessed with global instructions.  A
+ * null pointer stays null. */
+template <class T, class B>
+__device__ __forceinline__ T* rebase(const B* base, T* p)
+{
+    const char* b = reinterpret_cast<const char*>(base);
+    return reinterpret_cast<T*>(const_cast<char*>(b + (reinterpret_cast<uintptr_t>(p) - reinterpret_cast<uintptr_t>(b))));
+}
+
 /* one word of fix_split(v): hi (hiw = 1) or lo.  Both are integral floats below
  * 2^62 in magnitude, so one conversion of the selected one gives the same word. */
 __device__ __forceinline__ long long fix_word(float v, uint32_t hiw)
