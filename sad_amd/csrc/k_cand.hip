@@ -281,12 +281,36 @@ __device__ __forceinline__ void vpt_cell(int x, int y, const uint4* __restrict__
     };
 
     merge_cell(y * width + x);
+    /* The four neighbour cells' 32 ids, in the reference's order.  An id that is
+     * in the list after the cell's own 8 can add nothing: it is either still in
+     * the list when it comes up (skipped as already present) or was pushed out
+     * by a smaller key, and then its key is not below the list's last.  So only
+     * the other ids are tried, in order, each with the full check (an id that
+     * occurs in two neighbour cells is then caught as present).  The result is
+     * the reference's list, and a wave evaluates keys only as often as its
+     * lanes have such ids, instead of 32 times. */
     const int off[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    int ncell[4];
+    uint32_t todo = 0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        int nx = sad_clampi(x + off[j][0] * (int)jump, 0, width - 1);
-        int ny = sad_clampi(y + off[j][1] * (int)jump, 0, height - 1);
-        merge_cell(ny * width + nx);
+        const int nx = sad_clampi(x + off[j][0] * (int)jump, 0, width - 1);
+        const int ny = sad_clampi(y + off[j][1] * (int)jump, 0, height - 1);
+        ncell[j] = ny * width + nx;
+        const uint4 a = in0[ncell[j]];
+        const uint4 b = in1[ncell[j]];
+        const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (ids[k] < n && !contains<8>(bi, ids[k])) todo |= 1u << (8 * j + k);
+    }
+    while (todo != 0u) {
+        const uint32_t t = (uint32_t)__builtin_ctz(todo);
+        todo &= todo - 1u;
+        const uint32_t j = t >> 3, k = t & 7u;
+        const int cell = j == 0u ? ncell[0] : (j == 1u ? ncell[1] : (j == 2u ? ncell[2] : ncell[3]));
+        const uint32_t* plane = reinterpret_cast<const uint32_t*>(k < 4u ? in0 : in1);
+        try_insert(plane[4u * (uint32_t)cell + (k & 3u)]);
     }
 
     float rad = radius_scale * ((float)width / 1024.0f);
