@@ -541,7 +541,7 @@ ORC_API void orc_render_backward(const uint32_t* cand0, const uint32_t* cand1, c
  * Fixed point: a partial v becomes hi = trunc(v), lo = round-half-even of
  * (v - trunc(v)) * 2^40 (|v| saturated at 2^62, NaN taken as 0); sums of hi and
  * of lo are int64; the value is (float)((double)hi + (double)lo * 2^-40). */
-#define ORC_CHUNK 24
+#define ORC_CHUNK 20
 #define ORC_LANES 4
 #define ORC_TILE_ENTRIES (256 * SAD_K)
 

@@ -48,7 +48,7 @@
  * matters beyond the render: dL/dlogit = w g.(c - pred) cancels when one
  * candidate dominates a pixel, so ulp-level differences in pred would become
  * large relative differences in the gradient.  The per-pair derivative is
- * sad_grad_terms, the function the oracle evaluates (its square roots from the
+ * sad_grad_site and sad_grad_pair, the functions the oracle evaluates (its square roots from the
  * reproducible sad_rsqrt_nr), so per-pair values agree bit for bit as well.
  */
 #include <mutex>
@@ -66,9 +66,9 @@ constexpr uint16_t SLOT_OVERFLOW = 0xFFFEu;  /* valid id without a hash slot thi
 constexpr uint16_t ENTRY_FIRST = 0x8000u;    /* entry flag: first rank of its site in the pixel */
 
 #ifndef SAD_CHUNK
-#define SAD_CHUNK 24
+#define SAD_CHUNK 20
 #endif
-constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (6 per lane of a 4-lane group) */
+constexpr uint32_t CHUNK = SAD_CHUNK;       /* entries per chunk (5 per lane of a 4-lane group) */
 #ifndef SAD_CHUNK_LANES
 #define SAD_CHUNK_LANES 4
 #endif
@@ -176,14 +176,11 @@ __device__ __forceinline__ float d2_safe(const SiteTerms& t, float dx, float dy,
     return fmaxf(fmaf(t.p1.z, proj2, l2 * perp2) * iss, 1e-8f);
 }
 
-/* Derivative through one (pixel, candidate) pair: sad_grad_terms of sad_math.h
- * (gradients.cu:352-420) on the staged site terms. */
-__device__ __forceinline__ void candidate_grad(const SiteTerms& t, float ux, float uy, float w, float gr, float gg,
-                                               float gb, float pr, float pg_, float pb, float iss, float inv_scale,
-                                               float c[10])
+/* The staged terms of a site as sad_grad_site (sad_math.h, gradients.cu:352-420) takes them. */
+__device__ __forceinline__ SadGradSite grad_site(const SiteTerms& t, float iss, float inv_scale)
 {
-    sad_grad_terms(t.p0.x, t.p0.y, t.p0.z, t.p0.w, t.p1.x, t.p1.y, t.p1.z, t.p2.w, t.p2.x, t.p2.y, t.p2.z, ux, uy,
-                   w, gr, gg, gb, pr, pg_, pb, iss, inv_scale, c);
+    return sad_grad_site(t.p0.x, t.p0.y, t.p0.z, t.p0.w, t.p1.x, t.p1.y, t.p1.z, t.p2.w, t.p2.x, t.p2.y, t.p2.z, iss,
+                         inv_scale);
 }
 
 struct PixelArgs {
@@ -571,12 +568,13 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
         t.p0 = L.p0[s];
         t.p1 = L.p1[s];
         t.p2 = L.p2[s];
+        const SadGradSite gs = grad_site(t, A.iss, A.inv_scale);
         float acc[NV];
 #pragma unroll
         for (int j = 0; j < NV; ++j) acc[j] = 0.0f;
         for (uint32_t e = beg + part; e < end; e += CLANES) {
             const uint32_t ent = L.list[e];
-            const uint32_t p = (ent & 0x7FFFu) >> 3, k = ent & 7u;
+            const uint32_t p = (ent & 0x7FFFu) >> 3;
             const float4 pp = L.px[p];
             const float4 gq = L.pg[p];
             const float ex = MODE == MODE_STATS ? (float)(tx0 + (int)(p & 15u)) : pp.w;
@@ -603,8 +601,7 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             }
             if (MODE != MODE_STATS) {
                 float c[10];
-                candidate_grad(t, ex, ey, (&L.w[0][0])[ent & 0x7FFFu], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, A.iss,
-                               A.inv_scale, c);
+                sad_grad_pair(&gs, ex, ey, (&L.w[0][0])[ent & 0x7FFFu], gq.x, gq.y, gq.z, pp.x, pp.y, pp.z, c);
 #pragma unroll
                 for (int j = 0; j < 10; ++j) acc[j] += c[j];
             }

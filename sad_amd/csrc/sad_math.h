@@ -420,34 +420,71 @@ SAD_HD float sad_rsqrt_nr(float x)
 #define SAD_GRAD_SQRT_RSQ(x, sq, isd) do { (isd) = sad_rsqrt_nr(x); (sq) = (x) * (isd); } while (0)
 #endif
 
-SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
-                           float cr, float cg, float cb, float ux, float uy, float w, float gr, float gg, float gb,
-                           float pr, float pg, float pb, float iss, float inv_scale, float c[10])
+/* The site's share of sad_grad_terms: its terms, and the products of them that
+ * every pair of the site uses (so a caller that runs many pairs of one site
+ * forms them once). */
+typedef struct SadGradSite {
+    float px, py, tau, rn, dirx, diry, l2, cr, cg, cb, inv_scale;
+    float a1, a2;     /* (l1 - l2) dir.x, (l1 - l2) dir.y */
+    float l1s, l2s;   /* l1 / S^2, l2 / S^2 */
+    float ti;         /* tau / S^2 */
+    float b;          /* 2 (l1 - l2) / S^2 */
+} SadGradSite;
+
+SAD_HD SadGradSite sad_grad_site(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
+                                 float cr, float cg, float cb, float iss, float inv_scale)
+{
+    SadGradSite s;
+    s.px = px; s.py = py; s.tau = tau; s.rn = rn; s.dirx = dirx; s.diry = diry; s.l2 = l2;
+    s.cr = cr; s.cg = cg; s.cb = cb; s.inv_scale = inv_scale;
+    const float dl12 = l1 - l2;
+    s.a1 = dl12 * dirx;
+    s.a2 = dl12 * diry;
+    s.l1s = l1 * iss;
+    s.l2s = l2 * iss;
+    s.ti = tau * iss;
+    s.b = (2.0f * dl12) * iss;
+    return s;
+}
+
+/* The pair's share: the ten contributions of pixel (ux, uy) with weight w,
+ * dL/dpred g and blended colour pred, through site s. */
+SAD_HD void sad_grad_pair(const SadGradSite* s, float ux, float uy, float w, float gr, float gg, float gb, float pr,
+                          float pg, float pb, float c[10])
 {
     c[4] = w * gr;
     c[5] = w * gg;
     c[6] = w * gb;
-    float dlogit = c[4] * (cr - pr) + c[5] * (cg - pg) + c[6] * (cb - pb);
-    float dx = ux - px, dy = uy - py;
-    float diff2 = fmaf(dx, dx, dy * dy);
-    float proj = fmaf(dirx, dx, diry * dy);
-    float proj2 = proj * proj;
-    float perp2 = fmaxf(diff2 - proj2, 0.0f);
-    float d2s = fmaxf(fmaf(l1, proj2, l2 * perp2) * iss, 1e-8f);
+    const float dlogit = c[4] * (s->cr - pr) + c[5] * (s->cg - pg) + c[6] * (s->cb - pb);
+    const float dx = ux - s->px, dy = uy - s->py;
+    const float diff2 = fmaf(dx, dx, dy * dy);
+    const float proj = fmaf(s->dirx, dx, s->diry * dy);
+    const float proj2 = proj * proj;
+    const float perp2 = fmaxf(diff2 - proj2, 0.0f);
+    const float q = s->l2s * perp2;
+    const float d2s = fmaxf(fmaf(s->l1s, proj2, q), 1e-8f);
     float sq, isd;
     SAD_GRAD_SQRT_RSQ(d2s, sq, isd);
-    float dmix = sq - rn;
-    float dl12 = l1 - l2;
-    float cpos = dlogit * (tau * iss * isd);
-    c[0] = cpos * fmaf(dl12 * dirx, proj, l2 * dx);
-    c[1] = cpos * fmaf(dl12 * diry, proj, l2 * dy);
-    c[2] = -dlogit * dmix * tau;
-    c[3] = dlogit * tau * inv_scale;
-    float hq = -dlogit * tau * 0.5f * isd;
-    c[9] = hq * (fmaf(l1, proj2, -(l2 * perp2)) * iss);
-    float cd = 2.0f * dl12 * proj * iss;
-    c[7] = hq * (cd * dx);
-    c[8] = hq * (cd * dy);
+    const float dmix = sq - s->rn;
+    const float cpos = dlogit * (s->ti * isd);
+    c[0] = cpos * fmaf(s->a1, proj, s->l2 * dx);
+    c[1] = cpos * fmaf(s->a2, proj, s->l2 * dy);
+    const float dt = dlogit * s->tau;
+    c[2] = -(dt * dmix);
+    c[3] = dt * s->inv_scale;
+    const float hq = dt * (-0.5f * isd);
+    c[9] = hq * fmaf(s->l1s, proj2, -q);
+    const float hc = hq * (s->b * proj);
+    c[7] = hc * dx;
+    c[8] = hc * dy;
+}
+
+SAD_HD void sad_grad_terms(float px, float py, float tau, float rn, float dirx, float diry, float l1, float l2,
+                           float cr, float cg, float cb, float ux, float uy, float w, float gr, float gg, float gb,
+                           float pr, float pg, float pb, float iss, float inv_scale, float c[10])
+{
+    const SadGradSite s = sad_grad_site(px, py, tau, rn, dirx, diry, l1, l2, cr, cg, cb, iss, inv_scale);
+    sad_grad_pair(&s, ux, uy, w, gr, gg, gb, pr, pg, pb, c);
 }
 
 /* removalDeltaForSite (gradients.cu:82-100): change of |pred - tgt|^2 when the
