@@ -14,7 +14,7 @@
  * key (tau, e^a, e^-a, r/S) already evaluated, which saves two exp, one division
  * and the fp16 decodes per candidate and gives identical bits.
  */
-#include "batch.h"
+#include "fit_common.h"
 
 namespace sadh {
 
@@ -515,11 +515,21 @@ int sadh_jfa_flood(const uint32_t* in_cand0, uint32_t* out_cand0, const SadSite*
 {
     SADH_REQUIRE(in_cand0 && out_cand0 && sites && width > 0 && height > 0, "jfa_flood: bad arguments");
     SADH_REQUIRE(in_cand0 != out_cand0, "jfa_flood: input and output must differ");
-    hipLaunchKernelGGL(k_jfa_flood<false>, grid16(width, height), dim3(256), 0, to_hip(stream),
-                       reinterpret_cast<const uint4*>(in_cand0), reinterpret_cast<uint4*>(out_cand0), sites,
-                       (const float4*)nullptr, site_count, (int)step_size, inv_scale_sq, width, height,
-                       cand_downscale, target_width, target_height);
-    return launch_status("jfa_flood");
+    /* the key's per-site terms (two exps, a reciprocal, a product) are formed once per site into the library's
+     * stream workspace, not once per (cell, candidate); the keys are the same floats */
+    if (site_count == 0) {
+        hipLaunchKernelGGL(k_jfa_flood<false>, grid16(width, height), dim3(256), 0, to_hip(stream),
+                           reinterpret_cast<const uint4*>(in_cand0), reinterpret_cast<uint4*>(out_cand0), sites,
+                           (const float4*)nullptr, site_count, (int)step_size, inv_scale_sq, width, height,
+                           cand_downscale, target_width, target_height);
+        return launch_status("jfa_flood");
+    }
+    void* ws = nullptr;
+    SAD_TRY(stream_workspace(to_hip(stream), (size_t)site_count * sizeof(SadSearchSite), &ws));
+    SadSearchSite* search = static_cast<SadSearchSite*>(ws);
+    SAD_TRY(launch_search_sites(sites, search, site_count, inv_scale_sq, false, to_hip(stream)));
+    return launch_jfa_flood_search(in_cand0, out_cand0, search, site_count, step_size, inv_scale_sq, width, height,
+                                   cand_downscale, target_width, target_height, to_hip(stream));
 }
 
 int sadh_pack_candidate_sites(const SadSite* sites, SadPackedSite* packed, uint32_t site_count,
@@ -545,6 +555,9 @@ int sadh_update_candidates(const uint32_t* in_cand0, const uint32_t* in_cand1, u
     SADH_REQUIRE(in_cand0 != out_cand0 && in_cand1 != out_cand1, "update_candidates: in-place update");
     if (hilbert_probes > 0 && hilbert_window > 0)
         SADH_REQUIRE(hilbert_order && hilbert_pos, "update_candidates: hilbert buffers missing");
+    /* the keys are formed from the 16-byte packed sites at every use: at a million sites, 32-byte search-site
+     * records formed once per site (as the fit uses for its ~20k sites) double the footprint of the random
+     * injected-id gathers, and the pass measured slower with them (1856 against 1704 us at 4096^2) */
     PackedFetch f{reinterpret_cast<const uint4*>(sites), sqrtf(inv_scale_sq)};
     hipLaunchKernelGGL(k_vpt<PackedFetch>, grid16(width, height), dim3(256), 0, to_hip(stream),
                        reinterpret_cast<const uint4*>(in_cand0), reinterpret_cast<const uint4*>(in_cand1),

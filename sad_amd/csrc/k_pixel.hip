@@ -822,15 +822,27 @@ struct FixOut {
     uint32_t n;
 };
 
+/* One block per 256 sites.  The block first copies its sites' (lo, hi) pairs
+ * into LDS with consecutive lanes on consecutive 16-byte pairs (a site's NV
+ * pairs are contiguous in its line), so the line reads are coalesced; then
+ * each thread turns its site's pairs into floats and writes them. */
+template <uint32_t NV>
 __global__ __launch_bounds__(256) void k_fix_finalize(FixOut F)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= F.n) return;
-    const longlong2* L = reinterpret_cast<const longlong2*>(F.lines + (size_t)i * F.lstride);
+    __shared__ longlong2 pairs[256 * NV];
+    const uint32_t i0 = blockIdx.x * 256u;
+    const uint32_t ns = min(256u, F.n - i0);
+    for (uint32_t c = threadIdx.x; c < ns * NV; c += 256u) {
+        const uint32_t s = c / NV, j = c - s * NV;
+        pairs[c] = reinterpret_cast<const longlong2*>(F.lines + (size_t)(i0 + s) * F.lstride)[j];
+    }
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t >= ns) return;
+    const uint32_t i = i0 + t;
 #pragma unroll
-    for (uint32_t j = 0; j < 10u; ++j) {
-        if (j >= F.nv) break;
-        const longlong2 w = L[j];
+    for (uint32_t j = 0; j < NV; ++j) {
+        const longlong2 w = pairs[t * NV + j];
         const float v = fix_value(w.x, w.y);
         float* d = F.g[j] + (size_t)i * F.gstride;
         *d = F.assign ? v : *d + v;
@@ -844,7 +856,12 @@ __global__ __launch_bounds__(256) void k_fix_finalize(FixOut F)
 static int launch_fix_finalize(const FixOut& F, hipStream_t stream)
 {
     if (F.n == 0) return SADH_OK;
-    hipLaunchKernelGGL(k_fix_finalize, dim3(div_up(F.n, 256u)), dim3(256), 0, stream, F);
+    if (F.nv == 10u)
+        hipLaunchKernelGGL(k_fix_finalize<10u>, dim3(div_up(F.n, 256u)), dim3(256), 0, stream, F);
+    else if (F.nv == 8u)
+        hipLaunchKernelGGL(k_fix_finalize<8u>, dim3(div_up(F.n, 256u)), dim3(256), 0, stream, F);
+    else
+        return fail(SADH_ERR_INVALID, "finalize_sums: 8 or 10 values per site");
     return launch_status("finalize_sums");
 }
 
