@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsad_hip.so")
+# SAD_HIP_LIBRARY names another build of the same library, e.g. a schedule variant built with
+# `make -C sad_amd/csrc SCHEDULE=... OUT=...` for measurement; by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("SAD_HIP_LIBRARY") or os.path.join(_HERE, "libsad_hip.so")
 
 P = C.c_void_p
 U32 = C.c_uint32
