@@ -146,12 +146,12 @@ __device__ __forceinline__ SiteTerms site_terms(const SadSite* __restrict__ site
 }
 
 /* Insert-if-absent with linear probing, 8 probes: the slot, or SLOT_OVERFLOW. */
-template <class LDS, int H>
+template <class LDS, int H, int PROBES = 8>
 __device__ __forceinline__ uint16_t hash_insert(LDS& L, uint32_t id)
 {
     uint32_t h = (id * 2654435761u) & (uint32_t)(H - 1);
 #pragma unroll 1
-    for (int probe = 0; probe < 8; ++probe) {
+    for (int probe = 0; probe < PROBES; ++probe) {
         const uint32_t k = L.key[h];
         if (k == id) return (uint16_t)h;
         if (k == SAD_EMPTY) {
@@ -232,6 +232,13 @@ __device__ __forceinline__ void acc_add(const PixelArgs& A, uint32_t j, uint32_t
     else unsafeAtomicAdd(A.g[j] + (size_t)id * A.gstride, v);
 }
 
+/* Probes of the later rounds.  Every lane of a round probes the same window, so an id is placed and found
+ * consistently whatever its size; a tile that needs later rounds holds hundreds of distinct ids (a random
+ * candidate field), where most ids find the table full and a short window fails them sooner. */
+#ifndef SAD_PENDING_PROBES
+#define SAD_PENDING_PROBES 4
+#endif
+
 /* Round r's inserts of the ids still pending (slot SLOT_OVERFLOW); summed ids
  * leave the table (SLOT_INVALID).  Returns whether some id is still pending. */
 template <class LDS, int H>
@@ -241,7 +248,7 @@ __device__ __forceinline__ bool insert_pending(LDS& L, const uint32_t ids[8], ui
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (slots[k] == SLOT_OVERFLOW) {
-            slots[k] = hash_insert<LDS, H>(L, ids[k]);
+            slots[k] = hash_insert<LDS, H, SAD_PENDING_PROBES>(L, ids[k]);
             ovf = ovf || slots[k] == SLOT_OVERFLOW;
         } else {
             slots[k] = SLOT_INVALID;
