@@ -55,3 +55,21 @@ def test_fit_plan_and_capacity_error():
     rc = _lib.load().sadh_fit(C.byref(s), sites.ctypes.data, None, 1024, 1024, sites.ctypes.data, 10, None, 0,
                               C.byref(res))
     assert rc == -3 and "plan needs" in _lib.load().sadh_last_error().decode()
+
+
+def test_module_map_imports_the_headers(tmp_path):
+    """include/module.modulemap (the Swift/Clang-module view of the ABI) names headers that compile as a
+    module: a C translation unit that imports it sees the status codes and the site layout."""
+    import os
+    import shutil
+    import subprocess
+    clang = shutil.which("clang") or "/opt/rocm/llvm/bin/clang"
+    if not os.path.exists(clang):
+        pytest.skip("no clang to read the module map")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = tmp_path / "use.m"
+    src.write_text("@import SadHip;\n_Static_assert(sizeof(SadSite) == 40, \"Site\");\n"
+                   "int f(void) { return SADH_OK + SADH_ERR_CAPACITY; }\n")
+    r = subprocess.run([clang, "-fmodules", f"-fmodules-cache-path={tmp_path}", "-fsyntax-only", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
