@@ -631,13 +631,25 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
             const uint32_t u = i / (uint32_t)(2 * NV), r = i - u * (uint32_t)(2 * NV);
             const uint32_t j = r >> 1, hiw = r & 1u;
             const uint32_t s = L.occ[u];
+            /* the slot's words are read together, and the chunk partials four at a time (summed in chunk order):
+             * the flush is a chain of LDS round trips, not arithmetic */
             const uint32_t c = L.count[s];
-            if (c == 0u || !(L.p0[s].x >= 0.0f)) continue;
-            const uint32_t qa = L.cstart[s], qb = qa + (c + CHUNK - 1u) / CHUNK;
-            float v = 0.0f;
-            for (uint32_t q = qa; q < qb; ++q) v += L.part[q][j];
-            const long long word = fix_word(v, hiw);
+            const float sx = L.p0[s].x;
+            const uint32_t qa = L.cstart[s];
             const size_t id = L.key[s];
+            if (c == 0u || !(sx >= 0.0f)) continue;
+            const uint32_t qb = qa + (c + CHUNK - 1u) / CHUNK;
+            float v = 0.0f;
+            uint32_t q = qa;
+            for (; q + 4u <= qb; q += 4u) {
+                const float a0 = L.part[q][j], a1 = L.part[q + 1u][j], a2 = L.part[q + 2u][j], a3 = L.part[q + 3u][j];
+                v += a0;
+                v += a1;
+                v += a2;
+                v += a3;
+            }
+            for (; q < qb; ++q) v += L.part[q][j];
+            const long long word = fix_word(v, hiw);
             long long* dst;
             if (DST == DST_FIXSTATS) dst = A.sfix + id * FIX_STAT_STRIDE + 2u * j;
             else if (REMOVAL && j == 10u) dst = A.rfix + 2u * id;
