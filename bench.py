@@ -28,9 +28,13 @@ Other workloads (not the headline line):
                          (sad_amd.kernel_bench) with the CPU oracle on the same
                          inputs; the line carries every kernel's row.
   --workload batch64     64 images (about 60 MP) fitted as one batch for 500
-                         iterations, steady ticks replayed as captured
-                         hipGraphs (--graph-ticks 0: launched on the stream),
-                         and one at a time on the same inputs.
+                         iterations, ticks launched on the stream
+                         (--graph-ticks 1: steady ticks replayed as captured
+                         hipGraphs), and one at a time on the same inputs.
+  --workload smallbatch  --images N (256) images of --side S (128) px square,
+                         S*S/32 initial sites each, 100 iterations without
+                         densify: the batched path against the same fits one
+                         at a time, where single fits cannot fill the device.
   --workload serve       a Poisson stream of fit and render requests replayed
                          against the continuous-batching server: throughput
                          and p50/p99 latency per request kind.
@@ -166,14 +170,21 @@ def batch_workload(args) -> int:
     """--workload batch64: 64 images (sides log-uniform in [128, 2048], aspect
     log-uniform in [0.5, 2], bpp 1.0 initial sites), 500 iterations, fitted as
     one batch (sadh_fit_batch) and, for comparison, one image at a time
-    (sadh_fit) on the same inputs."""
+    (sadh_fit) on the same inputs.  --workload smallbatch: --images square
+    images of --side px with side*side/32 sites each, 100 iterations."""
     import numpy as np
     import torch
 
     from sad_amd import config, synth, trainer
     torch.cuda.set_device(0)
-    targets, sites = synth.batch_workload()
-    iters = 500
+    if args.workload == "batch64":
+        targets, sites = synth.batch_workload()
+        iters = 500
+    else:
+        side = args.side
+        targets = [synth.make_target(side, side, 100 + i, 40) for i in range(args.images)]
+        sites = [synth.init_sites_uniform(t, side * side // 32, 100 + i) for i, t in enumerate(targets)]
+        iters = 100
     opts = config.fit_options(iterations=iters, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
                               prune_percentile=0.0, log_freq=10**9)
     # the timed batches record no per-launch events (a profiled tick is never graph-replayed); one more
@@ -210,11 +221,14 @@ def batch_workload(args) -> int:
         cmp = sum(t.shape[0] * t.shape[1] for t in targets[:k]) / 1e6
         cpu = {"value": cmp * ci / cs, "unit": "MP-iters/s", "cores": threads, "kind": "port",
                "sample": f"first {ci} iterations of the first {k} images ({cmp:.2f} MP, {cs:.1f} s, OpenMP oracle)"}
+    workload = {"workload": args.workload}
+    if args.workload == "smallbatch":
+        workload.update(width=args.side, height=args.side)
     line = {
         "metric": "fit megapixel-iters/sec (fwd+bwd+Adam)", "value": round(value, 3), "unit": "MP-iters/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * loop_s / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batch64", "images": len(targets), "megapixels": round(mp, 3), "iterations": iters,
+        "config": {**workload, "images": len(targets), "megapixels": round(mp, 3), "iterations": iters,
                    "sites": int(sum(len(s0) for s0 in sites)), "graph_ticks": last["graph_ticks"],
                    "graph_captures": last["graph_captures"],
                    "jobs_bit_identical_to_single": bool(all_same),
@@ -268,13 +282,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64", "serve"])
+    ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64", "smallbatch", "serve"])
+    ap.add_argument("--images", type=int, default=256, help="smallbatch: images in the batch")
+    ap.add_argument("--side", type=int, default=128, help="smallbatch: image side in pixels")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate per second")
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph-ticks", type=int, default=1, choices=[0, 1],
-                    help="batch64: replay steady ticks as captured hipGraphs (1) or launch them on the stream (0)")
+    ap.add_argument("--graph-ticks", type=int, default=0, choices=[0, 1],
+                    help="batch64, smallbatch: replay steady ticks as captured hipGraphs (1) or launch them on the stream (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the fit path runs on one device)",
@@ -283,7 +299,7 @@ def main() -> int:
 
     if args.workload == "render4096":
         return kernel_workload(args)
-    if args.workload == "batch64":
+    if args.workload in ("batch64", "smallbatch"):
         return batch_workload(args)
     if args.workload == "serve":
         return serve_workload(args)

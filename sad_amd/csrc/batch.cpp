@@ -214,7 +214,8 @@ int setup_fit(const SadhFitOptions& base, FitRun& J, hipStream_t S, bool use_hil
     return SADH_OK;
 }
 
-void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_begin, uint32_t site_begin)
+void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_begin, uint32_t site_begin,
+              uint32_t pblock_begin, uint32_t sblock_begin)
 {
     std::memset(&d, 0, sizeof(d));
     d.target = J.d_target;
@@ -240,6 +241,7 @@ void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.tau_nbr = J.tau_nbr;
     d.removal = J.removal;
     d.rendered = J.d_render;
+    d.render = J.d_render;
     d.ssim_weight = J.ssim_weight;
     d.h_order = J.h_order;
     d.h_pos = J.h_pos;
@@ -253,6 +255,8 @@ void describe(const FitRun& J, BatchJob& d, uint32_t tile_begin, uint32_t ctile_
     d.cntiles = d.ctiles_x * div_up((uint32_t)J.ch, 16u);
     d.ctile_begin = ctile_begin;
     d.site_begin = site_begin;
+    d.pblock_begin = pblock_begin;
+    d.sblock_begin = sblock_begin;
 }
 
 /* The engine: running fits, their device table, streams and per-tick tables. */
@@ -266,7 +270,13 @@ struct Engine {
     DevArena tmem;                      /* device tables (a new one when the count grows) */
     BatchJob* d_jobs = nullptr;
     int d_jobs_cap = 0;
-    uint32_t total_tiles = 0, total_ctiles = 0, total_sites = 0;
+    uint32_t total_tiles = 0, total_ctiles = 0, total_sites = 0, total_pblocks = 0, total_sblocks = 0;
+    long long* d_logv = nullptr;        /* log ticks: (lo, hi, count, 0) per job, d_jobs_cap entries */
+    long long* h_logv = nullptr;        /* pinned host copy */
+    /* batched Hilbert tables: the tick's pair array, its sort scratch and histogram (hb_cap pairs) */
+    SadPair *hb_pairs = nullptr, *hb_scratch = nullptr;
+    uint32_t* hb_hist = nullptr;
+    uint32_t hb_cap = 0, hil_total = 0, hil_jobs = 0, hil_key_bits = 0;
     static constexpr int RING = 4;      /* per-tick step tables: pinned host + device slots */
     JobStep* h_steps = nullptr;
     JobStep* d_steps = nullptr;
@@ -292,6 +302,7 @@ struct Engine {
         if (steps_ready) (void)hipEventDestroy(steps_ready);
         if (h_steps) (void)hipHostFree(h_steps);
         if (d_steps) (void)hipFree(d_steps);
+        if (h_logv) (void)hipHostFree(h_logv);
     }
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t_origin).count(); }
@@ -332,16 +343,22 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(sp.cand));
         const int nj = (int)fits.size();
         std::vector<BatchJob> desc(nj);
-        total_tiles = total_ctiles = total_sites = 0;
+        total_tiles = total_ctiles = total_sites = total_pblocks = total_sblocks = 0;
         for (int j = 0; j < nj; ++j) {
-            describe(*fits[j], desc[j], total_tiles, total_ctiles, total_sites);
+            describe(*fits[j], desc[j], total_tiles, total_ctiles, total_sites, total_pblocks, total_sblocks);
             total_tiles += desc[j].ntiles;
             total_ctiles += desc[j].cntiles;
             total_sites += desc[j].cap;
+            total_pblocks += div_up((uint32_t)fits[j]->np, 256u);
+            total_sblocks += div_up(desc[j].cap, 256u);
         }
         if (nj > d_jobs_cap) {
             d_jobs_cap = std::max(nj, 2 * d_jobs_cap);
             SAD_TRY(tmem.alloc(&d_jobs, (size_t)d_jobs_cap));
+            SAD_TRY(tmem.alloc(&d_logv, 4u * (size_t)d_jobs_cap));
+            if (h_logv) (void)hipHostFree(h_logv);
+            h_logv = nullptr;
+            HIP_TRY(hipHostMalloc((void**)&h_logv, sizeof(long long) * 4u * (size_t)d_jobs_cap, hipHostMallocDefault));
         }
         if (nj > steps_cap) {
             if (h_steps) (void)hipHostFree(h_steps);
@@ -440,6 +457,7 @@ struct Engine {
          * field), the others' after it. */
         const bool split_upd = std::max(1, o.cand_update_passes) > 1;
         bool any_upd = false, any_dens_upd = false, any_prune = false, any_log = false, any_dens = false;
+        hil_total = hil_jobs = hil_key_bits = 0;
         for (int j = 0; j < nj; ++j) {
             FitRun& J = *fits[j];
             const int it = J.iter;
@@ -463,6 +481,17 @@ struct Engine {
             hs[j].b2c = 1.0f / (1.0f - sad_powi_f(o.beta2, t));
             hs[j].jump_index = J.jump_index;
             hs[j].vpt = (J.upd && (!split_upd || J.dens)) ? 1u : 0u;
+            hs[j].log = J.log ? 1u : 0u;
+            /* a Hilbert table is (re)built before the job's first update and after its site count changes */
+            const bool hil = use_hil && J.upd && (!J.hil_ready || J.hil_count != J.n);
+            hs[j].hil_begin = hil_total;
+            hs[j].hil_n = hil ? J.n : 0u;
+            hs[j].hil_bits = (uint32_t)sad_hilbert_bits(J.width, J.height);
+            if (hil) {
+                hil_total += J.n;
+                hil_jobs++;
+                hil_key_bits = std::max(hil_key_bits, 2u * hs[j].hil_bits);
+            }
         }
         for (int j = 0; j < nj; ++j) {
             hs[nj + j] = hs[j];
@@ -478,7 +507,7 @@ struct Engine {
             SAD_TRY(eager_tick(nj, ds, any_upd, any_dens_upd, any_prune));
         HIP_TRY(hipEventRecord(slot_done[slot], S));
         tick_no++;
-        return finish_tick(any_log, done);
+        return finish_tick(any_log, ds, done);
     }
 
     /* the VPT passes of this tick's update for the jobs whose flag is set in `steps`, on stream U */
@@ -511,11 +540,32 @@ struct Engine {
              * S; multi-pass updates run on S as well, those of non-densifying fits after the gradient */
             hipStream_t U = (any_dens_upd || split_upd) ? S : sp.cand;
             if (U != S) HIP_TRY(hipStreamWaitEvent(sp.cand, steps_ready, 0));
+            /* Two or more tables to build: one launch set for all of them, keyed (job << shift) | Hilbert
+             * index, when the job index and the widest Hilbert index fit 32 bits together */
+            uint32_t job_bits = 0;
+            while ((1u << job_bits) < (uint32_t)nj) job_bits++;
+            const bool hil_batched = hil_jobs >= 2 && hil_key_bits < 32u && job_bits + hil_key_bits <= 32u;
+            if (hil_batched) {
+                const uint32_t padded = div_up(hil_total, 1024u) * 1024u;
+                if (padded > hb_cap) {
+                    hb_cap = std::max(padded, 2u * hb_cap);
+                    SAD_TRY(tmem.alloc(&hb_pairs, hb_cap));
+                    SAD_TRY(tmem.alloc(&hb_scratch, hb_cap));
+                    SAD_TRY(tmem.alloc(&hb_hist, 256u * (hb_cap / 1024u)));
+                }
+                const uint32_t key_bits = job_bits + hil_key_bits;
+                const uint32_t mk = key_bits >= 32u ? 0xFFFFFFFFu : (1u << key_bits);
+                SAD_TRY(launch_hilbert_batch(d_jobs, (uint32_t)nj, ds, hil_total, hil_key_bits, mk, hb_pairs, hb_scratch,
+                                             hb_hist, U));
+            }
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.upd) continue;
                 J.n_upd++;
-                if (use_hil && (!J.hil_ready || J.hil_count != J.n)) {
+                if (hil_batched && use_hil && (!J.hil_ready || J.hil_count != J.n)) {
+                    J.hil_ready = true;
+                    J.hil_count = J.n;
+                } else if (use_hil && (!J.hil_ready || J.hil_count != J.n)) {
                     int bits = sad_hilbert_bits(J.width, J.height);
                     uint32_t pc = std::min(div_up(J.n, 1024u) * 1024u, J.hil_padded);
                     uint32_t mk = bits >= 16 ? 0xFFFFFFFFu : (1u << (2 * bits));
@@ -629,12 +679,24 @@ struct Engine {
     }
 
     /* the log renders and metrics, then the iteration count; retires finished fits */
-    int finish_tick(bool any_log, std::vector<std::unique_ptr<FitRun>>& done)
+    int finish_tick(bool any_log, const JobStep* ds, std::vector<std::unique_ptr<FitRun>>& done)
     {
         hipStream_t S = sp.main;
         const int nj = (int)fits.size();
         if (any_log) {
-            for (int j = 0; j < nj; ++j) {
+            int n_log = 0;
+            for (auto& f : fits) n_log += f->log ? 1 : 0;
+            /* Two or more logging jobs (a batch's jobs log on the same ticks): one render launch and one sum
+             * launch for all of them and one copy back, the same values as the per-job launches below (the
+             * same tiles and the same 256-pixel partials into an exact sum).  The SSIM metric keeps the
+             * per-job launches. */
+            const bool batched = n_log >= 2 && !o.ssim_metric;
+            if (batched) {
+                SAD_TRY(launch_render_batch(d_jobs, (uint32_t)nj, ds, total_tiles, S));
+                SAD_TRY(launch_log_sums_batch(d_jobs, (uint32_t)nj, ds, total_pblocks, total_sblocks, d_logv, S));
+                HIP_TRY(hipMemcpyAsync(h_logv, d_logv, sizeof(long long) * 4u * (size_t)nj, hipMemcpyDeviceToHost, S));
+            }
+            for (int j = 0; j < nj && !batched; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.log) continue;
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
@@ -652,6 +714,11 @@ struct Engine {
             for (int j = 0; j < nj; ++j) {
                 FitRun& J = *fits[j];
                 if (!J.log) continue;
+                if (batched) {
+                    J.sse[0] = h_logv[4 * j];
+                    J.sse[1] = h_logv[4 * j + 1];
+                    J.active_count = (uint32_t)h_logv[4 * j + 2];
+                }
                 float mse = fix_value_host(J.sse) / (std::max(1.0f, J.mask_sum) * 3.0f);
                 float psnr = mse > 0.0f ? 20.0f * log10f(1.0f / std::sqrt(mse)) : 100.0f;
                 if (J.io.log_psnr && J.n_logs < J.io.log_cap) J.io.log_psnr[J.n_logs] = psnr;

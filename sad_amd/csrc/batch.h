@@ -40,6 +40,7 @@ struct BatchJob {
     uint32_t* tau_nbr;         /* 8 ids per site: neighbour lists shared by the tau passes */
     long long* removal;        /* prune iterations: removal deltas, one fixed-point (lo, hi) pair per site */
     const float* rendered;     /* SSIM loss: render of the current state, written before the gradient */
+    float* render;             /* the same render buffer, written by the batched log renders */
     float ssim_weight;         /* SSIM loss weight (0: L2 only) */
     const uint32_t* h_order;   /* null: no Hilbert probes */
     const uint32_t* h_pos;
@@ -50,6 +51,8 @@ struct BatchJob {
     uint32_t tiles_x, ntiles, tile_begin;     /* 16x16 pixel tiles (gradient) */
     uint32_t ctiles_x, cntiles, ctile_begin;  /* 16x16 candidate-cell tiles (VPT) */
     uint32_t site_begin;
+    uint32_t pblock_begin;     /* 256-pixel row-major blocks (log PSNR sums) */
+    uint32_t sblock_begin;     /* 256-site blocks over the capacity (log active-site counts) */
 };
 
 /* Job j of the table with its pointers rebased on the table's own (kernel
@@ -74,6 +77,7 @@ __device__ __forceinline__ BatchJob job_global(const BatchJob* __restrict__ jobs
     J.tau_nbr = rebase(jobs, J.tau_nbr);
     J.removal = rebase(jobs, J.removal);
     J.rendered = rebase(jobs, J.rendered);
+    J.render = rebase(jobs, J.render);
     J.h_order = rebase(jobs, J.h_order);
     J.h_pos = rebase(jobs, J.h_pos);
     return J;
@@ -98,7 +102,24 @@ struct JobStep {
     float b1c, b2c;            /* Adam bias corrections 1/(1-beta^t), t = iteration + 1 */
     uint32_t jump_index;       /* the job's VPT pass index */
     uint32_t vpt;              /* 1: the job updates its candidates this tick */
+    uint32_t log;              /* 1: the job logs this tick (batched render, PSNR sum and active count) */
+    /* batched Hilbert tables: the job's sites occupy [hil_begin, hil_begin + hil_n) of the tick's pair array
+     * (hil_n = 0: no table this tick); hil_bits is the job's Hilbert grid (2^bits per side) */
+    uint32_t hil_begin, hil_n, hil_bits;
 };
+
+/* the job whose Hilbert range [hil_begin, hil_begin + hil_n) holds i < the ranges' total; the ranges are
+ * consecutive in job order, so it is the last job whose range starts at or before i */
+__device__ __forceinline__ uint32_t hil_find(const JobStep* __restrict__ steps, uint32_t njobs, uint32_t i)
+{
+    uint32_t lo = 0u, hi = njobs;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (steps[mid].hil_begin <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 /* scalars shared by every job of a batched launch */
 struct BatchStep {

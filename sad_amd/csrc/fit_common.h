@@ -148,6 +148,13 @@ int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_
 int launch_vpt_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
 int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, const BatchStep&, hipStream_t);
 int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
+/* log ticks: the renders of the jobs whose step flag `log` is set, then their squared-error sums and active-site
+ * counts as (lo, hi, count, 0) words per job */
+int launch_render_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, hipStream_t);
+int launch_log_sums_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, long long*, hipStream_t);
+/* the Hilbert tables of the jobs with hil_n > 0 (pairs, scratch: 1024-padded total; hist: 256 per 1024 pairs) */
+int launch_hilbert_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, uint32_t, SadPair*, SadPair*,
+                         uint32_t*, hipStream_t);
 
 /* a fixed-point (lo, hi) sum read back on the host, with the device's rounding (fix_value) */
 inline float fix_value_host(const long long w[2]) { return (float)((double)w[1] + (double)w[0] * 0x1p-40); }

@@ -742,6 +742,34 @@ __global__ __launch_bounds__(256) void k_pixel_grad_batch(const BatchJob* __rest
     pixel_tile<MODE_GRAD, REMOVAL, H, DST_FIX>(A, f - J.tile_begin, L);
 }
 
+/* Batched fit, log iterations: the render of every job whose step flag `log` is
+ * set, into its render buffer; the same tiles as the single-image render. */
+template <int H>
+__global__ __launch_bounds__(256) void k_pixel_render_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                            const JobStep* __restrict__ steps, uint32_t total_tiles)
+{
+    __shared__ PixelLds<MODE_RENDER, false, H> L;
+    const uint32_t f = xcd_tile(blockIdx.x, total_tiles);
+    const uint32_t j = batch_find(jobs, njobs, f, &BatchJob::tile_begin);
+    if (!steps[j].log) return;
+    const BatchJob J = job_global(jobs, j);
+    PixelArgs A = {};
+    A.cand0 = J.cur ? J.fb0 : J.fa0;
+    A.cand1 = J.cur ? J.fb1 : J.fa1;
+    A.sites = J.sites;
+    A.out = J.render;
+    A.iss = J.iss;
+    A.inv_scale = J.inv_scale;
+    A.n = J.n;
+    A.width = J.width;
+    A.height = J.height;
+    A.cw = J.cw;
+    A.ch = J.ch;
+    A.tiles_x = J.tiles_x;
+    A.ntiles = J.ntiles;
+    pixel_tile<MODE_RENDER, false, H, DST_SOA>(A, f - J.tile_begin, L);
+}
+
 /* ------------------------------------------------------------ launchers -- */
 
 #ifndef SAD_HASH_SLOTS
@@ -979,6 +1007,15 @@ int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_
         hipLaunchKernelGGL((k_pixel_grad_batch<false, HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs,
                            njobs, total_tiles);
     return launch_status("compute_gradients_batch");
+}
+
+int launch_render_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total_tiles,
+                        hipStream_t stream)
+{
+    if (total_tiles == 0) return SADH_OK;
+    hipLaunchKernelGGL((k_pixel_render_batch<HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs, njobs, steps,
+                       total_tiles);
+    return launch_status("render_batch");
 }
 
 /* computeSiteStatsSimpleKernel's float outputs (utils.cu:106-231): the statistics of site i stored into

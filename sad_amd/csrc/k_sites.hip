@@ -592,6 +592,49 @@ __global__ __launch_bounds__(256) void k_hilbert_order(const SadPair* __restrict
     pos[s] = i;
 }
 
+/* Batched fit: the Hilbert pairs of every job with hil_n > 0 in one array.  The
+ * key is (job << shift) | Hilbert index, so one stable sort leaves each job's
+ * sites in a range of their own, in the order the job's own sort gives them;
+ * entries past `total` are empty and sort last. */
+__global__ __launch_bounds__(256) void k_hilbert_pairs_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                             const JobStep* __restrict__ steps,
+                                                             SadPair* __restrict__ pairs, uint32_t total,
+                                                             uint32_t padded, uint32_t shift)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= padded) return;
+    uint2 out = make_uint2(SAD_EMPTY, 0u);
+    if (i < total) {
+        const uint32_t j = hil_find(steps, njobs, i);
+        const uint32_t local = i - steps[j].hil_begin;
+        const SadSite* sites = rebase(jobs, jobs[j].sites);
+        const int width = jobs[j].width, height = jobs[j].height;
+        const float2 pos = reinterpret_cast<const float2*>(sites + local)[0];
+        int px = (int)pos.x, py = (int)pos.y;
+        px = px < 0 ? 0 : (px >= width ? width - 1 : px);
+        py = py < 0 ? 0 : (py >= height ? height - 1 : py);
+        const uint32_t h = sad_hilbert_index((uint32_t)px, (uint32_t)py, (int)steps[j].hil_bits);
+        out = make_uint2((j << shift) | h, local);
+    }
+    reinterpret_cast<uint2*>(pairs)[i] = out;
+}
+
+/* each job's range of the sorted pairs into its order / position tables */
+__global__ __launch_bounds__(256) void k_hilbert_order_batch(const BatchJob* __restrict__ jobs,
+                                                             const JobStep* __restrict__ steps,
+                                                             const SadPair* __restrict__ sorted, uint32_t total,
+                                                             uint32_t shift)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint2 p = reinterpret_cast<const uint2*>(sorted)[i];
+    const uint32_t j = p.x >> shift;
+    const uint32_t r = i - steps[j].hil_begin;
+    /* the descriptor holds the tables read-only for the VPT pass; this is their writer */
+    const_cast<uint32_t*>(rebase(jobs, jobs[j].h_order))[r] = p.y;
+    const_cast<uint32_t*>(rebase(jobs, jobs[j].h_pos))[p.y] = r;
+}
+
 /* ------------------------------------------------------------- metrics --- */
 
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
@@ -602,9 +645,9 @@ __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
  * fixed-point kind (k_sse_fix) adds the partial as an exact (lo, hi) pair, so
  * the fit's PSNR log does not depend on the blocks' order. */
 __device__ __forceinline__ float sse_block(const float* __restrict__ r, const float* __restrict__ t,
-                                           const float* __restrict__ mask, uint32_t np, float* part)
+                                           const float* __restrict__ mask, uint32_t np, float* part, uint32_t block)
 {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t i = block * 256u + threadIdx.x;
     float v = 0.0f;
     if (i < np && mask[i] > 0.0f) {
         float d0 = r[3 * i] - t[3 * i], d1 = r[3 * i + 1] - t[3 * i + 1], d2 = r[3 * i + 2] - t[3 * i + 2];
@@ -621,7 +664,7 @@ __global__ __launch_bounds__(256) void k_sse_fix(const float* __restrict__ r, co
                                                  uint32_t np)
 {
     __shared__ float part[4];
-    const float v = sse_block(r, t, mask, np, part);
+    const float v = sse_block(r, t, mask, np, part, blockIdx.x);
     if (threadIdx.x < 2u) {
         long long lo, hi;
         fix_split(v, &lo, &hi);
@@ -634,12 +677,45 @@ __global__ void k_zero_pair(long long* p)
     if (threadIdx.x < 2u) p[threadIdx.x] = 0;
 }
 
+/* Batched fit, log iterations: the first total_pblocks blocks add each logging
+ * job's squared error, block by block as k_sse_fix (the same 256-pixel blocks,
+ * so the same exact sum), into out[4 j] and out[4 j + 1]; the rest count its
+ * active sites into out[4 j + 2].  out is cleared by k_clear before. */
+__global__ __launch_bounds__(256) void k_log_sums_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                        const JobStep* __restrict__ steps, uint32_t total_pblocks,
+                                                        long long* __restrict__ out)
+{
+    __shared__ float part[4];
+    if (blockIdx.x < total_pblocks) {
+        const uint32_t j = batch_find(jobs, njobs, blockIdx.x, &BatchJob::pblock_begin);
+        if (!steps[j].log) return;
+        const BatchJob J = job_global(jobs, j);
+        const uint32_t np = (uint32_t)J.width * (uint32_t)J.height;
+        const float v = sse_block(J.render, J.target, J.mask, np, part, blockIdx.x - J.pblock_begin);
+        if (threadIdx.x < 2u) {
+            long long lo, hi;
+            fix_split(v, &lo, &hi);
+            fix_atomic_add(out + 4u * j + threadIdx.x, threadIdx.x ? hi : lo);
+        }
+        return;
+    }
+    const uint32_t sb = blockIdx.x - total_pblocks;
+    const uint32_t j = batch_find(jobs, njobs, sb, &BatchJob::sblock_begin);
+    if (!steps[j].log) return;
+    const uint32_t n = jobs[j].n;
+    const SadSite* sites = rebase(jobs, jobs[j].sites);
+    const uint32_t i = (sb - jobs[j].sblock_begin) * 256u + threadIdx.x;
+    const bool a = i < n && sites[i].pos_x >= 0.0f;
+    const unsigned long long b = __ballot(a);
+    if ((threadIdx.x & 63u) == 0u && b) fix_atomic_add(out + 4u * j + 2u, (long long)__popcll(b));
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sse(const float* __restrict__ r, const float* __restrict__ t,
                                              const float* __restrict__ mask, T* __restrict__ acc, uint32_t np)
 {
     __shared__ float part[4];
-    const float v = sse_block(r, t, mask, np, part);
+    const float v = sse_block(r, t, mask, np, part, blockIdx.x);
     if (threadIdx.x == 0) unsafeAtomicAdd(acc, (T)v);
 }
 
@@ -1160,6 +1236,16 @@ int launch_unpack_inference(const SadInferenceSite* in, const SadInferenceQuant*
     return launch_status("unpack_inference_sites");
 }
 
+int launch_log_sums_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total_pblocks,
+                          uint32_t total_sblocks, long long* out, hipStream_t stream)
+{
+    SAD_TRY(launch_clear_fix(out, 4u * (size_t)njobs, stream));
+    if (total_pblocks + total_sblocks == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_log_sums_batch, dim3(total_pblocks + total_sblocks), dim3(256), 0, stream, jobs, njobs, steps,
+                       total_pblocks, out);
+    return launch_status("log_sums_batch");
+}
+
 int launch_count_active(const SadSite* sites, uint32_t* count, uint32_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, stream, count);
@@ -1235,6 +1321,21 @@ int launch_hilbert_pairs(const SadSite* sites, SadPair* pairs, uint32_t n, uint3
     hipLaunchKernelGGL(k_hilbert_pairs, dim3(div_up(padded, 256u)), dim3(256), 0, stream, sites, pairs, n, padded,
                        width, height, bits);
     return launch_status("hilbert_pairs");
+}
+
+int launch_hilbert_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total, uint32_t shift,
+                         uint32_t max_key_exclusive, SadPair* pairs, SadPair* scratch, uint32_t* hist,
+                         hipStream_t stream)
+{
+    if (total == 0) return SADH_OK;
+    const uint32_t padded = div_up(total, 1024u) * 1024u;
+    hipLaunchKernelGGL(k_hilbert_pairs_batch, dim3(padded / 256u), dim3(256), 0, stream, jobs, njobs, steps, pairs,
+                       total, padded, shift);
+    SAD_TRY(launch_status("hilbert_pairs_batch"));
+    SAD_TRY(launch_radix_sort(pairs, scratch, hist, padded, max_key_exclusive, stream));
+    hipLaunchKernelGGL(k_hilbert_order_batch, dim3(div_up(total, 256u)), dim3(256), 0, stream, jobs, steps, pairs,
+                       total, shift);
+    return launch_status("hilbert_order_batch");
 }
 
 int launch_hilbert_order(const SadPair* sorted, uint32_t* order, uint32_t* pos, uint32_t n, hipStream_t stream)
