@@ -82,6 +82,7 @@ struct FitRun {
     int n_logs = 0, n_dens = 0, n_prune = 0, n_upd = 0;
     uint32_t active_count = 0;
     double submit_s = 0.0, admit_s = 0.0;
+    uint32_t site_begin = 0;           /* offset of the job's sites in the flattened site range */
     /* this tick */
     bool upd = false, dens = false, prune = false, log = false;
 
@@ -277,6 +278,10 @@ struct Engine {
     SadPair *hb_pairs = nullptr, *hb_scratch = nullptr;
     uint32_t* hb_hist = nullptr;
     uint32_t hb_cap = 0, hil_total = 0, hil_jobs = 0, hil_key_bits = 0;
+    /* retirement: the finishing jobs' sites gathered on the device and copied into pinned memory */
+    SadSite* d_stage = nullptr;
+    SadSite* h_stage = nullptr;
+    uint32_t stage_cap = 0;
     static constexpr int RING = 4;      /* per-tick step tables: pinned host + device slots */
     JobStep* h_steps = nullptr;
     JobStep* d_steps = nullptr;
@@ -303,6 +308,7 @@ struct Engine {
         if (h_steps) (void)hipHostFree(h_steps);
         if (d_steps) (void)hipFree(d_steps);
         if (h_logv) (void)hipHostFree(h_logv);
+        if (h_stage) (void)hipHostFree(h_stage);
     }
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t_origin).count(); }
@@ -351,6 +357,7 @@ struct Engine {
             total_sites += desc[j].cap;
             total_pblocks += div_up((uint32_t)fits[j]->np, 256u);
             total_sblocks += div_up(desc[j].cap, 256u);
+            fits[j]->site_begin = desc[j].site_begin;
         }
         if (nj > d_jobs_cap) {
             d_jobs_cap = std::max(nj, 2 * d_jobs_cap);
@@ -482,6 +489,7 @@ struct Engine {
             hs[j].jump_index = J.jump_index;
             hs[j].vpt = (J.upd && (!split_upd || J.dens)) ? 1u : 0u;
             hs[j].log = J.log ? 1u : 0u;
+            hs[j].retire = it + 1 >= J.iterations ? 1u : 0u;
             /* a Hilbert table is (re)built before the job's first update and after its site count changes */
             const bool hil = use_hil && J.upd && (!J.hil_ready || J.hil_count != J.n);
             hs[j].hil_begin = hil_total;
@@ -741,12 +749,39 @@ struct Engine {
             finished = finished || f->iter >= f->iterations;
         }
         if (finished) {
+            /* Two or more fits finishing: one gather of their sites, one copy per run of consecutive finishing
+             * jobs into pinned memory and one synchronisation, instead of a blocking copy per fit */
+            int n_fin = 0;
+            for (auto& f : fits) n_fin += f->iter >= f->iterations ? 1 : 0;
+            const bool gather = n_fin >= 2;
+            if (gather) {
+                if (total_sites > stage_cap) {
+                    stage_cap = std::max(total_sites, 2u * stage_cap);
+                    SAD_TRY(tmem.alloc(&d_stage, stage_cap));
+                    if (h_stage) (void)hipHostFree(h_stage);
+                    h_stage = nullptr;
+                    HIP_TRY(hipHostMalloc((void**)&h_stage, sizeof(SadSite) * stage_cap, hipHostMallocDefault));
+                }
+                SAD_TRY(launch_gather_sites_batch(d_jobs, (uint32_t)nj, ds, total_sites, d_stage, S));
+                for (int j = 0; j < nj;) {
+                    if (fits[j]->iter < fits[j]->iterations) { ++j; continue; }
+                    int e = j + 1;
+                    while (e < nj && fits[e]->iter >= fits[e]->iterations) ++e;
+                    const uint32_t b = fits[j]->site_begin, end = fits[e - 1]->site_begin + (uint32_t)fits[e - 1]->cap;
+                    HIP_TRY(hipMemcpyAsync(h_stage + b, d_stage + b, sizeof(SadSite) * (end - b), hipMemcpyDeviceToHost,
+                                           S));
+                    j = e;
+                }
+            }
             HIP_TRY(hipStreamSynchronize(S));
             HIP_TRY(hipStreamSynchronize(sp.cand));
             std::vector<std::unique_ptr<FitRun>> keep;
             for (auto& f : fits) {
                 if (f->iter < f->iterations) { keep.push_back(std::move(f)); continue; }
-                HIP_TRY(hipMemcpy(f->io.sites_io, f->d_sites, sizeof(SadSite) * f->cap, hipMemcpyDeviceToHost));
+                if (gather)
+                    std::memcpy(f->io.sites_io, h_stage + f->site_begin, sizeof(SadSite) * f->cap);
+                else
+                    HIP_TRY(hipMemcpy(f->io.sites_io, f->d_sites, sizeof(SadSite) * f->cap, hipMemcpyDeviceToHost));
                 SAD_TRY(finish_renders(*f, S));
                 done.push_back(std::move(f));
             }

@@ -106,6 +106,7 @@ struct JobStep {
     /* batched Hilbert tables: the job's sites occupy [hil_begin, hil_begin + hil_n) of the tick's pair array
      * (hil_n = 0: no table this tick); hil_bits is the job's Hilbert grid (2^bits per side) */
     uint32_t hil_begin, hil_n, hil_bits;
+    uint32_t retire;           /* 1: the job's last iteration (its sites are gathered for the copy back) */
 };
 
 /* the job whose Hilbert range [hil_begin, hil_begin + hil_n) holds i < the ranges' total; the ranges are

@@ -152,6 +152,8 @@ int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32
  * counts as (lo, hi, count, 0) words per job */
 int launch_render_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, hipStream_t);
 int launch_log_sums_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, long long*, hipStream_t);
+/* the sites of the jobs whose step flag `retire` is set, at their site_begin offsets of `out` */
+int launch_gather_sites_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, SadSite*, hipStream_t);
 /* the Hilbert tables of the jobs with hil_n > 0 (pairs, scratch: 1024-padded total; hist: 256 per 1024 pairs) */
 int launch_hilbert_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, uint32_t, SadPair*, SadPair*,
                          uint32_t*, hipStream_t);

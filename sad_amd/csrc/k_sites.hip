@@ -619,6 +619,22 @@ __global__ __launch_bounds__(256) void k_hilbert_pairs_batch(const BatchJob* __r
     reinterpret_cast<uint2*>(pairs)[i] = out;
 }
 
+/* Batched fit, last iteration: the sites of every job whose step flag `retire`
+ * is set, at the job's offset of the flattened site range (site_begin). */
+__global__ __launch_bounds__(256) void k_gather_sites_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                            const JobStep* __restrict__ steps, uint32_t total,
+                                                            SadSite* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t j = batch_find(jobs, njobs, i, &BatchJob::site_begin);
+    if (!steps[j].retire) return;
+    const float2* src = reinterpret_cast<const float2*>(rebase(jobs, jobs[j].sites) + (i - jobs[j].site_begin));
+    float2* dst = reinterpret_cast<float2*>(out + i);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dst[k] = src[k];
+}
+
 /* each job's range of the sorted pairs into its order / position tables */
 __global__ __launch_bounds__(256) void k_hilbert_order_batch(const BatchJob* __restrict__ jobs,
                                                              const JobStep* __restrict__ steps,
@@ -1336,6 +1352,15 @@ int launch_hilbert_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* st
     hipLaunchKernelGGL(k_hilbert_order_batch, dim3(div_up(total, 256u)), dim3(256), 0, stream, jobs, steps, pairs,
                        total, shift);
     return launch_status("hilbert_order_batch");
+}
+
+int launch_gather_sites_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total, SadSite* out,
+                              hipStream_t stream)
+{
+    if (total == 0) return SADH_OK;
+    hipLaunchKernelGGL(k_gather_sites_batch, dim3(div_up(total, 256u)), dim3(256), 0, stream, jobs, njobs, steps, total,
+                       out);
+    return launch_status("gather_sites_batch");
 }
 
 int launch_hilbert_order(const SadPair* sorted, uint32_t* order, uint32_t* pos, uint32_t n, hipStream_t stream)

@@ -133,3 +133,21 @@ def test_batch64_full_size(orc):
     small = sorted(range(64), key=lambda i: targets[i].shape[0] * targets[i].shape[1])[:3]
     for i in small:
         assert_same_fit(out["jobs"][i], orc.fit(targets[i], sites[i], dict(opts, n_sites=sites[i].shape[0])))
+
+
+def test_many_small_jobs_match_single_fits():
+    """40 small images in one batch: the log ticks (one render and one sum
+    launch for every logging job), the Hilbert tables (one sort of all jobs'
+    sites, keyed by job) and the retirement (one gather and copy) run batched.
+    Every job ends with the sites, PSNR log and active count of its single fit."""
+    specs = [(48 + 8 * (i % 5), 40 + 8 * (i % 3), 300 + i, 64 + 16 * (i % 4)) for i in range(40)]
+    tg, st = _jobs(specs)
+    opts = config.fit_options(iterations=50, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
+                              prune_percentile=0.0, log_freq=20)
+    out = trainer.fit_batch(tg, st, opts)
+    for i, (t, s0) in enumerate(zip(tg, st)):
+        b = out["jobs"][i]
+        assert b["n_logs"] == 4
+        g = trainer.fit(t, s0, opts)
+        assert_same_fit(b, g)
+        assert b["active_count"] == g["active_count"] == int(np.sum(b["sites"][: s0.shape[0], 0] >= 0))
