@@ -28,15 +28,29 @@ struct AdamArgs {
     uint32_t n;
 };
 
-/* one site's update from its normalised gradient g[10] */
-__device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const float g[10])
+/* a site and its Adam moments as read by adam_apply */
+struct AdamIn {
+    float2 s0, s1, s2, s3, s4;
+    float4 a0, a1, a2, a3, a4;
+};
+
+__device__ __forceinline__ AdamIn adam_load(const AdamArgs& A, uint32_t i)
 {
-    float2* sp = reinterpret_cast<float2*>(A.sites + i);
-    float2 s0 = sp[0], s1 = sp[1], s2 = sp[2], s3 = sp[3], s4 = sp[4];
+    const float2* sp = reinterpret_cast<const float2*>(A.sites + i);
+    const float4* ap = reinterpret_cast<const float4*>(A.st + i);
+    AdamIn in;
+    in.s0 = sp[0]; in.s1 = sp[1]; in.s2 = sp[2]; in.s3 = sp[3]; in.s4 = sp[4];
+    in.a0 = ap[0]; in.a1 = ap[1]; in.a2 = ap[2]; in.a3 = ap[3]; in.a4 = ap[4];
+    return in;
+}
+
+/* one site's update from its normalised gradient g[10], given the site and moments read by adam_load */
+__device__ __forceinline__ void adam_apply(const AdamArgs& A, uint32_t i, const AdamIn& in, const float g[10])
+{
+    float2 s0 = in.s0, s1 = in.s1, s2 = in.s2, s3 = in.s3, s4 = in.s4;
     /* s0 = pos, s1 = (log_tau, radius), s2 = (r, g), s3 = (b, dir.x), s4 = (dir.y, log_aniso) */
     if (s0.x < 0.0f) return;
-    float4* ap = reinterpret_cast<float4*>(A.st + i);
-    float4 a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3], a4 = ap[4];
+    float4 a0 = in.a0, a1 = in.a1, a2 = in.a2, a3 = in.a3, a4 = in.a4;
     /* a0 = (m_pos.x, m_pos.y, v_pos, m_log_tau), a1 = (v_log_tau, m_radius, v_radius, m_r),
      * a2 = (m_g, m_b, v_r, v_g), a3 = (v_b, m_dir.x, m_dir.y, v_dir), a4 = (m_la, v_la, pad, pad) */
     const float b1 = A.beta1, b2 = A.beta2, ob1 = 1.0f - A.beta1, ob2 = 1.0f - A.beta2;
@@ -87,8 +101,16 @@ __device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const f
     a4.y = b2 * a4.y + ob2 * (g[9] * g[9]);
     s4.y = sad_clampf(s4.y - A.lr_aniso * (a4.x * b1c) / (sqrtf(a4.y * b2c) + eps), -4.0f, 4.0f);
 
+    float2* sp = reinterpret_cast<float2*>(A.sites + i);
+    float4* ap = reinterpret_cast<float4*>(A.st + i);
     sp[0] = s0; sp[1] = s1; sp[2] = s2; sp[3] = s3; sp[4] = s4;
     ap[0] = a0; ap[1] = a1; ap[2] = a2; ap[3] = a3; ap[4] = a4;
+}
+
+/* one site's update from its normalised gradient g[10] */
+__device__ __forceinline__ void adam_site(const AdamArgs& A, uint32_t i, const float g[10])
+{
+    adam_apply(A, i, adam_load(A, i), g);
 }
 
 /* site i's ten gradients times norm, with log_tau replaced by `tau`, then
@@ -112,6 +134,31 @@ __device__ __forceinline__ void take_gradients(const AdamArgs& A, uint32_t i, fl
     for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? tau : A.g[j][i * A.gstride]) * A.norm;
 #pragma unroll
     for (int j = 0; j < 10; ++j) A.g[j][i * A.gstride] = 0.0f;
+}
+
+/* take_gradients for fixed-point lines in two steps: the line's words are read
+ * first (grad_line_load), so the read can be issued early, and converted and
+ * cleared later (grad_line_take) */
+struct GradLine {
+    longlong2 d[10];
+};
+
+__device__ __forceinline__ GradLine grad_line_load(const AdamArgs& A, uint32_t i)
+{
+    const longlong2* line = reinterpret_cast<const longlong2*>(A.gfix + (size_t)i * FIX_GRAD_STRIDE);
+    GradLine L;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) L.d[j] = line[j];
+    return L;
+}
+
+__device__ __forceinline__ void grad_line_take(const AdamArgs& A, uint32_t i, const GradLine& L, float tau, float g[10])
+{
+#pragma unroll
+    for (int j = 0; j < 10; ++j) g[j] = ((j == 2) ? tau : fix_value(L.d[j].x, L.d[j].y)) * A.norm;
+    longlong2* line = reinterpret_cast<longlong2*>(A.gfix + (size_t)i * FIX_GRAD_STRIDE);
+#pragma unroll
+    for (int j = 0; j < 10; ++j) line[j] = make_longlong2(0, 0);
 }
 
 template <bool CLEAR>
@@ -344,11 +391,19 @@ __global__ __launch_bounds__(256) void k_tau_list_34_adam(const uint4* __restric
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = t >> 3, lane = t & 7u;
     if (i >= A.n) return;
+    /* the group's first lane reads the site, its moments and its gradient line before the tau passes, so
+     * those reads wait alongside the passes' first reads instead of after the passes */
+    AdamIn in = {};
+    GradLine gl = {};
+    if (lane == 0u) {
+        in = adam_load(A, i);
+        gl = grad_line_load(A, i);
+    }
     const float tau = tau_two_passes8(i, lane, nbr, graw, gin, lambda, 1u);
     if (lane != 0u) return;
     float g[10];
-    take_gradients(A, i, tau, g);
-    adam_site(A, i, g);
+    grad_line_take(A, i, gl, tau, g);
+    adam_apply(A, i, in, g);
 }
 
 __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c0, const uint4* __restrict__ c1,
