@@ -113,34 +113,95 @@ def write_sites_txt(sites: np.ndarray, width: int, height: int, path: str) -> No
             f.write(" ".join(f"{float(v):.9g}" for v in row) + "\n")
 
 
+def _mitchell(x: np.ndarray) -> np.ndarray:
+    """Mitchell-Netravali cubic (B = C = 1/3), support 2"""
+    x = np.abs(x)
+    return np.where(x < 1.0, (16.0 + x * x * (21.0 * x - 36.0)) / 18.0,
+                    np.where(x < 2.0, (32.0 + x * (-60.0 + x * (36.0 - 7.0 * x))) / 18.0, 0.0))
+
+
+def _catmull_rom(x: np.ndarray) -> np.ndarray:
+    """Catmull-Rom cubic (B = 0, C = 1/2), support 2"""
+    x = np.abs(x)
+    return np.where(x < 1.0, 1.0 - x * x * (2.5 - 1.5 * x),
+                    np.where(x < 2.0, 2.0 - x * (4.0 + x * (0.5 * x - 2.5)), 0.0))
+
+
+def resample_matrix(n_in: int, n_out: int) -> np.ndarray:
+    """[n_out][n_in] weights of one axis of the reference's resize (stb_image_resize2's easy API, which loadImage
+    and loadMask call at sad_cuda.cu:157, 196): pixel centres at i + 0.5; a downscale (s = n_out / n_in < 1)
+    weighs input pixel i for output j by the Mitchell filter at (j + 0.5) - (i + 0.5) s, in output pixels, an
+    upscale by the Catmull-Rom filter at (i + 0.5) - (j + 0.5) / s, in input pixels; each output's weights are
+    normalised to sum 1 and the taps beyond the image are moved onto its edge pixel (clamp)."""
+    m = np.zeros((n_out, n_in), np.float64)
+    s = n_out / n_in
+    j = np.arange(n_out, dtype=np.float64)[:, None]
+    if n_out < n_in:
+        r = 2.0 / s
+        i = np.arange(int(np.floor(-r)) - 1, n_in + int(np.ceil(r)) + 1)[None, :]
+        w = _mitchell((j + 0.5) - (i + 0.5) * s) * s
+    else:
+        i = np.arange(-3, n_in + 3)[None, :]
+        w = _catmull_rom((i + 0.5) - (j + 0.5) / s)
+    w = w / w.sum(axis=1, keepdims=True)
+    cols = np.clip(i[0], 0, n_in - 1)
+    for k, c in enumerate(cols):
+        m[:, c] += w[:, k]
+    return m
+
+
+def resize_plane(a: np.ndarray, out_w: int, out_h: int) -> np.ndarray:
+    """[H][W](...) float64 planes resized separably with resample_matrix"""
+    h, w = a.shape[:2]
+    my, mx = resample_matrix(h, out_h), resample_matrix(w, out_w)
+    return np.einsum("yh,hw...,xw->yx...", my, a, mx, optimize=True)
+
+
+def srgb_to_linear(v: np.ndarray) -> np.ndarray:
+    v = np.asarray(v, np.float64)
+    return np.where(v <= 0.04045, v / 12.92, ((v + 0.055) / 1.055) ** 2.4)
+
+
+def linear_to_srgb(v: np.ndarray) -> np.ndarray:
+    v = np.clip(np.asarray(v, np.float64), 0.0, 1.0)
+    return np.where(v <= 0.0031308, v * 12.92, 1.055 * v ** (1.0 / 2.4) - 0.055)
+
+
+def resize_rgb8_srgb(img: np.ndarray, out_w: int, out_h: int) -> np.ndarray:
+    """uint8 [H][W][3] -> uint8 [out_h][out_w][3], filtered in linear light as stbir_resize_uint8_srgb"""
+    lin = resize_plane(srgb_to_linear(np.asarray(img, np.float64) / 255.0), out_w, out_h)
+    return np.floor(linear_to_srgb(lin) * 255.0 + 0.5).astype(np.uint8)
+
+
+def resize_gray8_linear(img: np.ndarray, out_w: int, out_h: int) -> np.ndarray:
+    """uint8 [H][W] -> uint8 [out_h][out_w], filtered on the 8-bit values as stbir_resize_uint8_linear"""
+    v = resize_plane(np.asarray(img, np.float64) / 255.0, out_w, out_h)
+    return np.floor(np.clip(v, 0.0, 1.0) * 255.0 + 0.5).astype(np.uint8)
+
+
 def load_image(path: str, max_dim: int = 0) -> np.ndarray:
-    """float32 [H][W][3] in [0, 1].  Images larger than max_dim are scaled in
-    linear light (the reference resizes with stb's sRGB-aware filter; that
-    resampling is not reproduced bit for bit)."""
+    """float32 [H][W][3] in [0, 1] (loadImage, sad_cuda.cu:137-180).  An image whose larger side exceeds
+    max_dim is first scaled to round(side * max_dim / max(W, H)) pixels per side (resize_rgb8_srgb)."""
     from PIL import Image
-    im = Image.open(path).convert("RGB")
-    w, h = im.size
+    a = np.asarray(Image.open(path).convert("RGB"), np.uint8)
+    h, w = a.shape[:2]
     if max_dim > 0 and max(w, h) > max_dim:
-        sc = max_dim / max(w, h)
-        ow, oh = max(1, int(round(w * sc))), max(1, int(round(h * sc)))
-        a = np.asarray(im, np.float32) / 255.0
-        lin = np.where(a <= 0.04045, a / 12.92, ((a + 0.055) / 1.055) ** 2.4)
-        ch = [np.asarray(Image.fromarray(lin[..., c].astype(np.float32), mode="F").resize((ow, oh),
-                                                                                         Image.Resampling.BICUBIC))
-              for c in range(3)]
-        lin = np.clip(np.stack(ch, -1), 0.0, 1.0)
-        srgb = np.where(lin <= 0.0031308, lin * 12.92, 1.055 * lin ** (1 / 2.4) - 0.055)
-        return to_8bit(srgb).astype(np.float32) / np.float32(255.0)
-    return np.asarray(im, np.uint8).astype(np.float32) / np.float32(255.0)
+        sc = np.float32(max_dim) / np.float32(max(w, h))
+        # std::round of the float32 product: halves away from zero
+        ow = max(1, int(np.floor(np.float64(np.float32(w) * sc) + 0.5)))
+        oh = max(1, int(np.floor(np.float64(np.float32(h) * sc) + 0.5)))
+        a = resize_rgb8_srgb(a, ow, oh)
+    return a.astype(np.float32) / np.float32(255.0)
 
 
 def load_mask(path: str, width: int, height: int) -> np.ndarray:
-    """float32 [H][W] = grey / 255, resized (bilinear) to the target size (loadMask, sad_cuda.cu:182-218)."""
+    """float32 [H][W] = grey / 255 (loadMask, sad_cuda.cu:182-218), resized to the target size first when it
+    differs (resize_gray8_linear)."""
     from PIL import Image
-    im = Image.open(path).convert("L")
-    if im.size != (width, height):
-        im = im.resize((width, height), Image.Resampling.BILINEAR)
-    return np.asarray(im, np.uint8).astype(np.float32) / np.float32(255.0)
+    a = np.asarray(Image.open(path).convert("L"), np.uint8)
+    if a.shape != (height, width):
+        a = resize_gray8_linear(a, width, height)
+    return a.astype(np.float32) / np.float32(255.0)
 
 
 def to_8bit(image: np.ndarray) -> np.ndarray:
