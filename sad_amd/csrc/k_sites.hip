@@ -339,6 +339,8 @@ template <typename TR, typename TI>
 __device__ __forceinline__ float tau_two_passes8(uint32_t i, uint32_t lane, const uint4* __restrict__ nbr,
                                                  TR graw, TI gin, float lambda, uint32_t stride)
 {
+    /* site i's own raw value and list are read first, alongside each other */
+    const float raw_i = (float)graw[i * stride];
     const uint4 a = nbr[2u * i], b = nbr[2u * i + 1u];
     const uint32_t ids[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     uint32_t mine = ids[0];
@@ -361,7 +363,7 @@ __device__ __forceinline__ float tau_two_passes8(uint32_t i, uint32_t lane, cons
             cnt += 1u;
         }
     }
-    return ((float)graw[i * stride] + lambda * sum) / (1.0f + lambda * (float)cnt);
+    return (raw_i + lambda * sum) / (1.0f + lambda * (float)cnt);
 }
 
 /* passes 1 and 2 from lists built during the gradient launch: reads the
@@ -377,7 +379,7 @@ __global__ __launch_bounds__(256) void k_tau_list_12_built(const uint4* __restri
     if (i >= n) return;                        /* whole groups of 8 leave together */
     const float v = tau_two_passes8(i, lane, nbr, g2, g2, lambda, 1u);
     if (lane == 0u) {
-        raw_copy[i] = (float)g2[i];
+        raw_copy[i] = (float)g2[i];   /* the same word tau_two_passes8 read first: no second wait */
         gout[i] = v;
     }
 }
