@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void k_tau_diffuse(const uint4* __restrict__ c
 __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t pass,
                                                    const JobStep* __restrict__ steps, BatchStep st)
 {
-    const uint32_t f = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t jx = batch_find(jobs, njobs, f, &BatchJob::site_begin);
     const BatchJob J = job_global(jobs, jx);
     const JobStep js = steps[jx];
@@ -450,9 +450,13 @@ __global__ __launch_bounds__(256) void k_tau_batch(const BatchJob* __restrict__ 
         A.lr_pos = st.lr_pos; A.lr_tau = st.lr_tau; A.lr_radius = st.lr_radius; A.lr_color = st.lr_color;
         A.lr_dir = st.lr_dir; A.lr_aniso = st.lr_aniso; A.beta1 = st.beta1; A.beta2 = st.beta2; A.eps = st.eps;
         A.b1c = js.b1c; A.b2c = js.b2c; A.norm = J.norm; A.maxx = J.maxx; A.maxy = J.maxy; A.n = J.n;
+        /* the site, its moments and its gradient line are read before the last pass's gathers */
+        const AdamIn in = adam_load(A, i);
+        const GradLine gl = grad_line_load(A, i);
+        const float tau = tau_from_list(i, nbr, (const float*)J.tau_raw, (const float*)J.tau_tmp, js.lambda);
         float g[10];
-        take_gradients(A, i, tau_from_list(i, nbr, (const float*)J.tau_raw, (const float*)J.tau_tmp, js.lambda), g);
-        adam_site(A, i, g);
+        grad_line_take(A, i, gl, tau, g);
+        adam_apply(A, i, in, g);
     }
 }
 
