@@ -10,14 +10,21 @@
  * over pixels, cells or sites exactly as the reference's CUDA threads do; where a
  * reference kernel has a data race (jfaSeedKernel: several sites writing one
  * cell) the deterministic rule the HIP library implements is stated and used.
- * Gradient-type sums (fp32 atomics in the reference, order-nondeterministic) are
- * accumulated here in double, and the sum of absolute contributions is returned
- * alongside so tests can bound fp32 summation error.
+ * Gradient-type sums (fp32 atomics in the reference, order-nondeterministic) come
+ * in two forms: accumulated in double, with the sum of absolute contributions
+ * alongside so tests can bound the float-output entry points' fp32 summation
+ * error; and, for the fit, in the library's own order (per 16x16 tile and site,
+ * chunks of ORC_CHUNK entries, partials added in exact fixed point), so the
+ * oracle's fit matches the GPU's bit for bit.
  *
  * Floating-point primitives come from sad_amd/csrc/sad_math.h, compiled with
  * -ffp-contract=off, so candidate lists and score keys are bit-comparable with
- * the GPU.  sad_math.h's exp/log/fp16 routines are themselves pinned against
- * libm and numpy by tests/test_oracle_primitives.py.
+ * the GPU.  Because the primitives are shared, they and the functions built on
+ * them are pinned independently of the GPU: exp/log/pow/fp16 against libm and
+ * numpy error bounds, render and gradients against closed forms and float64
+ * central differences, site statistics, removal deltas, the fixed-point sums and
+ * the bpp solver against brute force, conservation laws and hand-counted
+ * schedules (tests/test_oracle.py, tests/test_oracle_sums.py).
  */
 #include <stdint.h>
 #include <stdlib.h>
