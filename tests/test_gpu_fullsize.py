@@ -9,7 +9,8 @@ the same inputs; the oracle's OpenMP loops finish each in seconds on the host.
     field, then jump 1 with Hilbert probes and injection): bit-exact lists;
   * the softmax render: within 1e-5;
   * the tiled gradient backward: per-site relative bar;
-  * Adam with the gradient clear: bit-exact sites and state."""
+  * Adam with the gradient clear: bit-exact sites and state;
+  * one tau diffusion pass through the entry point: bit-exact."""
 import os
 
 import numpy as np
@@ -141,3 +142,18 @@ def test_gradient_and_adam(gpu, orc, c3, field):
     assert np.array_equal(gpu.get(sites).view(np.uint32), rs.view(np.uint32))
     assert np.array_equal(gpu.get(state).view(np.uint32), rst.view(np.uint32))
     assert all(np.all(gpu.get(g) == 0) for g in grads)
+
+
+def test_tau_diffuse_full_size(gpu, orc, c3, field):
+    """sadh_tau_diffuse at 1M sites, one in 20 inactive: every output equals the oracle's."""
+    c0, c1 = gpu.get(field["a0"], np.uint32), gpu.get(field["a1"], np.uint32)
+    sites = c3["sites"].copy()
+    sites[::20, 0] = -1.0
+    rng = np.random.default_rng(23)
+    graw = rng.normal(0, 1, N).astype(np.float32)
+    gin = rng.normal(0, 1, N).astype(np.float32)
+    out = gpu.zeros((N,))
+    gpu.call("sadh_tau_diffuse", field["a0"], field["a1"], gpu.put(sites), gpu.put(graw), gpu.put(gin), out, N, 0.05,
+             W, H, 1)
+    ref = orc.tau_diffuse(c0, c1, sites, graw, gin, 0.05, W, H)
+    assert np.array_equal(gpu.get(out).view(np.uint32), ref.view(np.uint32))
