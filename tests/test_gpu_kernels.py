@@ -125,7 +125,8 @@ def test_hilbert_tables(gpu, orc):
 # -------------------------------------------------------------- radix sort
 
 @pytest.mark.parametrize("n,kind", [(1, "full"), (1000, "full"), (1024, "dup"), (1025, "full"), (65536, "16bit"),
-                                    (262144, "full"), (300001, "score"), (70000, "sorted"), (70000, "reverse")])
+                                    (20000, "score"), (40000, "reverse"), (262144, "full"), (300001, "score"),
+                                    (70000, "sorted"), (70000, "reverse")])
 def test_radix_sort_bitexact(gpu, orc, n, kind):
     rng = np.random.default_rng(n)
     if kind == "full":
