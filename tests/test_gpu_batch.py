@@ -151,3 +151,23 @@ def test_many_small_jobs_match_single_fits():
         g = trainer.fit(t, s0, opts)
         assert_same_fit(b, g)
         assert b["active_count"] == g["active_count"] == int(np.sum(b["sites"][: s0.shape[0], 0] >= 0))
+
+
+def test_masked_batch_matches_single_fits():
+    """Jobs with masks (partly zero, partly fractional): the batched log sums count only pixels with mask > 0
+    and divide by each job's own mask count, as the single fit does; every job ends as its single fit."""
+    specs = [(80, 64, 410, 200), (72, 90, 411, 240), (96, 48, 412, 160)]
+    tg, st = _jobs(specs)
+    rng = np.random.default_rng(7)
+    masks = []
+    for (w, h, _, _) in specs:
+        m = rng.uniform(0.0, 1.0, (h, w)).astype(np.float32)
+        m[: h // 3] = 0.0
+        masks.append(m)
+    opts = config.fit_options(iterations=40, target_bpp=0.0, densify_enabled=0, densify_percentile=0.0,
+                              prune_percentile=0.0, log_freq=10)
+    out = trainer.fit_batch(tg, st, opts, masks=masks)
+    for i, (t, s0) in enumerate(zip(tg, st)):
+        g = trainer.fit(t, s0, opts, mask=masks[i])
+        assert_same_fit(out["jobs"][i], g)
+        assert out["jobs"][i]["n_logs"] == 5
