@@ -189,7 +189,7 @@ def batch_workload(args) -> int:
                               prune_percentile=0.0, log_freq=10**9)
     # the timed batches record no per-launch events (a profiled tick is never graph-replayed); one more
     # batch with profile_gradients times the batched gradient launches for the roofline
-    bopts = dict(opts, graph_ticks=args.graph_ticks)
+    bopts = dict(opts, graph_ticks=args.graph_ticks, autotune=args.autotune)
     mp = sum(t.shape[0] * t.shape[1] for t in targets) / 1e6
     for _ in range(args.warmup):
         trainer.fit_batch(targets[:4], sites[:4], bopts)
@@ -231,6 +231,8 @@ def batch_workload(args) -> int:
         "config": {**workload, "images": len(targets), "megapixels": round(mp, 3), "iterations": iters,
                    "sites": int(sum(len(s0) for s0 in sites)), "graph_ticks": last["graph_ticks"],
                    "graph_captures": last["graph_captures"],
+                   "autotune": {"hash_slots": last["tuned_hash_slots"], "tau_block": last["tuned_tau_block"]}
+                   if args.autotune else None,
                    "jobs_bit_identical_to_single": bool(all_same),
                    "mean_final_psnr_db": round(float(np.mean([j["final_psnr"] for j in last["jobs"]])), 3)},
         "single_image_path": {"value": round(single, 3), "unit": "MP-iters/s", "seconds": round(single_s, 3),
@@ -283,6 +285,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fit1024", choices=["fit1024", "fit256", "render4096", "batch64", "smallbatch", "serve"])
+    ap.add_argument("--autotune", type=int, default=0, choices=[0, 1],
+                    help="batch64, smallbatch: time steady ticks under each result-neutral launch schedule "
+                         "(tile hash size, tau block size) and keep the fastest")
     ap.add_argument("--images", type=int, default=256, help="smallbatch: images in the batch")
     ap.add_argument("--side", type=int, default=128, help="smallbatch: image side in pixels")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")

@@ -343,6 +343,11 @@ typedef struct SadhFitOptions {
                                          candidate update, densify, prune, log or profiling) as a captured
                                          hipGraph of its gradient and tau/Adam launches; the same results.
                                          No reference counterpart; sadh_fit ignores it */
+    int autotune;                     /* sadh_fit_batch and the server: 1 times steady iterations (from the
+                                         41st) under each launch schedule that leaves every result unchanged
+                                         (128- or 64-slot tile hash, 256- or 64-thread tau blocks) and keeps
+                                         the fastest for the rest of the run.  No reference counterpart;
+                                         sadh_fit ignores it */
 } SadhFitOptions;
 
 typedef struct SadhFitResult {
@@ -392,6 +397,7 @@ typedef struct SadhBatchResult {
     double grad_bytes;                /* summed algorithmic bytes of those launches */
     double total_pixels;              /* sum of W*H over the jobs */
     int graph_ticks, graph_captures;  /* graph_ticks: iterations replayed as a graph, graphs captured */
+    int tuned_hash_slots, tuned_tau_block; /* autotune: the schedule kept (0, 0 when the run did not tune) */
 } SadhBatchResult;
 
 int sadh_fit_plan(const SadhFitOptions* opts, int width, int height, float* densify_percentile,

@@ -38,7 +38,7 @@ class FitOptions(C.Structure):
         ("lr_dir", F32), ("lr_aniso", F32), ("beta1", F32), ("beta2", F32), ("eps", F32),
         ("log_freq", I32), ("profile_gradients", I32), ("init_gradient_alpha", F32), ("init_log_tau", F32),
         ("init_radius", F32), ("ssim_weight", F32), ("ssim_metric", I32), ("packed_psnr", I32),
-        ("graph_ticks", I32),
+        ("graph_ticks", I32), ("autotune", I32),
     ]
 
 
@@ -76,6 +76,7 @@ class BatchResult(C.Structure):
     _fields_ = [
         ("train_seconds", C.c_double), ("grad_kernel_ms", C.c_double), ("grad_launches", I32),
         ("grad_bytes", C.c_double), ("total_pixels", C.c_double), ("graph_ticks", I32), ("graph_captures", I32),
+        ("tuned_hash_slots", I32), ("tuned_tau_block", I32),
     ]
 
 

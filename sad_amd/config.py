@@ -57,7 +57,7 @@ def fit_options(cfg: dict | None = None, **over) -> dict:
         lr_color=c["LR_COLOR_BASE"], lr_dir=c["LR_DIR_BASE"], lr_aniso=c["LR_ANISO_BASE"],
         beta1=c["BETA1"], beta2=c["BETA2"], eps=c["EPS"], log_freq=1000, profile_gradients=0,
         init_gradient_alpha=1.0, init_log_tau=c["INIT_LOG_TAU"], init_radius=c["INIT_RADIUS"],
-        ssim_weight=0.0, ssim_metric=0, packed_psnr=0, graph_ticks=0,
+        ssim_weight=0.0, ssim_metric=0, packed_psnr=0, graph_ticks=0, autotune=0,
     )
     for k, v in over.items():
         if k not in o:

@@ -296,14 +296,40 @@ struct Engine {
      * count changed by densify, or (with the SSIM render) a candidate field flipped on the host. */
     hipGraphExec_t tick_exec[RING] = {};
     uint64_t tick_epoch[RING] = {};
+    int tick_sched[RING] = {};          /* the schedule a slot's graph was captured with (index + 1) */
     uint64_t epoch = 1;
     int graph_replays = 0, graph_captures = 0;
+    /* The schedule of a tick's launches: the tile-hash size of the batched gradient and the block size of the
+     * batched tau passes.  Neither changes a result.  With autotune, steady ticks take the candidates in turn,
+     * each timed by the stream period it takes, and the fastest is kept.  (Graph replays are not a candidate:
+     * they measured slower in every configuration, and trying one costs a capture per ring slot.) */
+    struct Schedule {
+        int hash_slots;
+        uint32_t tau_block;
+    };
+    static constexpr int N_SCHEDULES = 3;
+    static constexpr int TUNE_SAMPLES = 5;              /* ticks per candidate; the first is not counted */
+    /* Tuning starts after this many ticks: a fit's first iterations, over the random initial field, cost several
+     * times a converged one and fall quickly, which would favour whichever candidate runs later.  The rounds
+     * alternate their order (ABC, CBA, ...), so a steady trend weighs on every candidate alike. */
+    static constexpr uint64_t TUNE_START = 40;
+    static constexpr Schedule CANDIDATES[N_SCHEDULES] = {{128, 256u}, {64, 256u}, {128, 64u}};
+    int sched_id = 0;                   /* index into CANDIDATES */
+    bool tuning = false;
+    int tune_ticks = 0;
+    int tune_open = -1;                 /* the tuning tick whose end is the next tick's start */
+    /* Per tuning tick, an event where its stream work starts and one where the next tick's starts: the whole
+     * period the schedule costs the stream, its launch overheads included.  Nothing waits on them until the
+     * last, so stream launches stay queued ahead of the device as in untimed ticks. */
+    hipEvent_t tev[2 * N_SCHEDULES * TUNE_SAMPLES] = {};
+    int tuned_hash_slots = 0, tuned_tau_block = 0;
 
     ~Engine()
     {
         for (auto& g : tick_exec) if (g) (void)hipGraphExecDestroy(g);
         for (auto e : gev) (void)hipEventDestroy(e);
         for (auto& e : slot_done) if (e) (void)hipEventDestroy(e);
+        for (auto& e : tev) if (e) (void)hipEventDestroy(e);
         if (steps_ready) (void)hipEventDestroy(steps_ready);
         if (h_steps) (void)hipHostFree(h_steps);
         if (d_steps) (void)hipFree(d_steps);
@@ -324,6 +350,10 @@ struct Engine {
         HIP_TRY(hipEventCreateWithFlags(&sp.sites_ready, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&steps_ready, hipEventDisableTiming));
         for (auto& e : slot_done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        sched_id = 0;
+        tuning = o.autotune != 0;
+        if (tuning)
+            for (auto& e : tev) HIP_TRY(hipEventCreate(&e));
         return SADH_OK;
     }
 
@@ -387,7 +417,7 @@ struct Engine {
     /* the launches of a steady iteration: the SSIM renders (when the loss has that term), the batched
      * gradient and the batched tau diffusion + Adam; every per-iteration value comes from the device
      * step table `ds` and the job table, so the same launches replay as a graph */
-    int steady_launches(int nj, const JobStep* ds, hipStream_t S)
+    int steady_launches(int nj, const JobStep* ds, hipStream_t S, const Schedule& k)
     {
         if (o.ssim_weight > 0.0f)
             for (int j = 0; j < nj; ++j) {
@@ -395,9 +425,9 @@ struct Engine {
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
                                       J.ch, S));
             }
-        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, false, S));
+        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, false, S, k.hash_slots));
         BatchStep st = adam_step();
-        return launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, st, S);
+        return launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, st, S, k.tau_block);
     }
 
     BatchStep adam_step() const
@@ -409,18 +439,18 @@ struct Engine {
         return st;
     }
 
-    /* replay slot `slot`'s graph of the steady launches, capturing it first when its epoch is old */
+    /* replay slot `slot`'s graph of the steady launches, capturing it first when its epoch or schedule is old */
     int graph_tick(int slot, int nj, const JobStep* ds)
     {
         hipStream_t S = sp.main;
-        if (!tick_exec[slot] || tick_epoch[slot] != epoch) {
+        if (!tick_exec[slot] || tick_epoch[slot] != epoch || tick_sched[slot] != sched_id + 1) {
             if (tick_exec[slot]) {
                 HIP_TRY(hipGraphExecDestroy(tick_exec[slot]));
                 tick_exec[slot] = nullptr;
             }
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed));
-            const int rc = steady_launches(nj, ds, S);
+            const int rc = steady_launches(nj, ds, S, CANDIDATES[sched_id]);
             const hipError_t ec = hipStreamEndCapture(S, &g);
             if (rc != SADH_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -431,6 +461,7 @@ struct Engine {
             (void)hipGraphDestroy(g);
             HIP_TRY(ei);
             tick_epoch[slot] = epoch;
+            tick_sched[slot] = sched_id + 1;
             graph_captures++;
         }
         HIP_TRY(hipGraphLaunch(tick_exec[slot], S));
@@ -505,17 +536,56 @@ struct Engine {
             hs[nj + j] = hs[j];
             hs[nj + j].vpt = (split_upd && fits[j]->upd && !fits[j]->dens) ? 1u : 0u;
         }
+        const bool steady = !any_upd && !any_dens && !any_prune && !any_log && !o.profile_gradients;
+        if (tune_open >= 0) {
+            HIP_TRY(hipEventRecord(tev[2 * tune_open + 1], S));
+            tune_open = -1;
+            if (tune_ticks == N_SCHEDULES * TUNE_SAMPLES) SAD_TRY(choose_schedule());
+        }
+        const bool tune_now = steady && tuning && tick_no >= TUNE_START;
+        if (tune_now) HIP_TRY(hipEventRecord(tev[2 * tune_ticks], S));
         HIP_TRY(hipMemcpyAsync(ds, hs, sizeof(JobStep) * 2 * nj, hipMemcpyHostToDevice, S));
         if (any_upd) HIP_TRY(hipEventRecord(steps_ready, S));
 
-        const bool steady = o.graph_ticks && !any_upd && !any_dens && !any_prune && !any_log && !o.profile_gradients;
-        if (steady)
+        if (tune_now) {
+            SAD_TRY(steady_launches(nj, ds, S, CANDIDATES[tune_candidate(tune_ticks)]));
+            tune_open = tune_ticks++;
+        } else if (steady && o.graph_ticks)
             SAD_TRY(graph_tick(slot, nj, ds));
         else
             SAD_TRY(eager_tick(nj, ds, any_upd, any_dens_upd, any_prune));
         HIP_TRY(hipEventRecord(slot_done[slot], S));
         tick_no++;
         return finish_tick(any_log, ds, done);
+    }
+
+    /* the candidate of tuning tick i: round i / N in order, then reversed, alternately */
+    static int tune_candidate(int i)
+    {
+        const int r = i / N_SCHEDULES, k = i % N_SCHEDULES;
+        return (r & 1) ? N_SCHEDULES - 1 - k : k;
+    }
+
+    /* after TUNE_SAMPLES tuning ticks of each candidate: the schedule with the least total period over its
+     * ticks after the first (which may include a graph capture or a first launch) */
+    int choose_schedule()
+    {
+        const int last = 2 * (N_SCHEDULES * TUNE_SAMPLES) - 1;
+        HIP_TRY(hipEventSynchronize(tev[last]));
+        double total[N_SCHEDULES] = {};
+        for (int i = N_SCHEDULES; i < N_SCHEDULES * TUNE_SAMPLES; ++i) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]));
+            total[tune_candidate(i)] += ms;
+        }
+        int best = 0;
+        for (int i = 1; i < N_SCHEDULES; ++i)
+            if (total[i] < total[best]) best = i;
+        sched_id = best;
+        tuning = false;
+        tuned_hash_slots = CANDIDATES[best].hash_slots;
+        tuned_tau_block = (int)CANDIDATES[best].tau_block;
+        return SADH_OK;
     }
 
     /* the VPT passes of this tick's update for the jobs whose flag is set in `steps`, on stream U */
@@ -653,7 +723,8 @@ struct Engine {
                 SAD_TRY(launch_render(J.c0(), J.c1(), J.d_render, J.d_sites, J.iss, J.n, J.width, J.height, J.cw,
                                       J.ch, S));
             }
-        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, any_prune, S));
+        SAD_TRY(launch_gradients_batch(d_jobs, (uint32_t)nj, total_tiles, any_prune, S,
+                                       CANDIDATES[sched_id].hash_slots));
         if (o.profile_gradients) {
             HIP_TRY(hipEventRecord(gev.back(), S));
             double b = 0.0;
@@ -666,7 +737,8 @@ struct Engine {
         SAD_TRY(flip(late, S));
         for (uint32_t j : late) fits[j]->cur ^= 1u;
 
-        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, adam_step(), S));
+        SAD_TRY(launch_tau_adam_batch(d_jobs, (uint32_t)nj, ds, total_sites, adam_step(), S,
+                                      CANDIDATES[sched_id].tau_block));
 
         for (int j = 0; j < nj; ++j) {
             FitRun& J = *fits[j];
@@ -885,6 +957,8 @@ static int fit_batch_impl(const SadhFitOptions& o, SadhBatchJob* ios, int n_jobs
         bres->total_pixels = px;
         bres->graph_ticks = E.graph_replays;
         bres->graph_captures = E.graph_captures;
+        bres->tuned_hash_slots = E.tuned_hash_slots;
+        bres->tuned_tau_block = E.tuned_tau_block;
     }
     return SADH_OK;
 }

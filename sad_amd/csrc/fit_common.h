@@ -144,9 +144,10 @@ int launch_densify_pairs_fix(const SadSite*, const long long*, SadPair*, uint32_
 int launch_prune_pairs_fix(const SadSite*, const long long*, SadPair*, uint32_t, float, uint32_t, hipStream_t);
 
 /* batched fit (one launch over every job) */
-int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t);
+int launch_gradients_batch(const BatchJob*, uint32_t, uint32_t, bool, hipStream_t, int hash_slots = 128);
 int launch_vpt_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, uint32_t, const BatchStep&, hipStream_t);
-int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, const BatchStep&, hipStream_t);
+int launch_tau_adam_batch(const BatchJob*, uint32_t, const JobStep*, uint32_t, const BatchStep&, hipStream_t,
+                          uint32_t block = 256);
 int launch_update_jobs(BatchJob*, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, hipStream_t);
 /* log ticks: the renders of the jobs whose step flag `log` is set, then their squared-error sums and active-site
  * counts as (lo, hi, count, 0) words per job */

@@ -996,16 +996,25 @@ int launch_gradients_fix(const uint32_t* cand0, const uint32_t* cand1, const flo
     return launch_pixel<MODE_GRAD, false, DST_FIX>(A, stream, "compute_gradients(fix)");
 }
 
-int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
-                           hipStream_t stream)
+template <int H>
+static void launch_grad_batch_h(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
+                                hipStream_t stream)
 {
-    if (total_tiles == 0) return SADH_OK;
     if (removal)
-        hipLaunchKernelGGL((k_pixel_grad_batch<true, HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs, njobs,
+        hipLaunchKernelGGL((k_pixel_grad_batch<true, H>), dim3(total_tiles), dim3(256), 0, stream, jobs, njobs,
                            total_tiles);
     else
-        hipLaunchKernelGGL((k_pixel_grad_batch<false, HASH_SLOTS>), dim3(total_tiles), dim3(256), 0, stream, jobs,
-                           njobs, total_tiles);
+        hipLaunchKernelGGL((k_pixel_grad_batch<false, H>), dim3(total_tiles), dim3(256), 0, stream, jobs, njobs,
+                           total_tiles);
+}
+
+/* hash_slots: the tile hash size, 64 or the build's (HASH_SLOTS); every result is the same for either */
+int launch_gradients_batch(const BatchJob* jobs, uint32_t njobs, uint32_t total_tiles, bool removal,
+                           hipStream_t stream, int hash_slots)
+{
+    if (total_tiles == 0) return SADH_OK;
+    if (hash_slots == 64 && HASH_SLOTS != 64) launch_grad_batch_h<64>(jobs, njobs, total_tiles, removal, stream);
+    else launch_grad_batch_h<HASH_SLOTS>(jobs, njobs, total_tiles, removal, stream);
     return launch_status("compute_gradients_batch");
 }
 

@@ -1093,13 +1093,15 @@ int launch_tau_adam(const uint32_t* c0, const uint32_t* c1, float* raw, float* t
     return launch_status("tau_diffuse_adam");
 }
 
+/* block: threads per block, 64 or 256 (the same results) */
 int launch_tau_adam_batch(const BatchJob* jobs, uint32_t njobs, const JobStep* steps, uint32_t total_sites,
-                          const BatchStep& st, hipStream_t stream)
+                          const BatchStep& st, hipStream_t stream, uint32_t block)
 {
     if (total_sites == 0) return SADH_OK;
+    if (block != 64u) block = 256u;
     for (uint32_t pass = 0; pass < 4u; ++pass)
-        hipLaunchKernelGGL(k_tau_batch, dim3(div_up(total_sites, 256u)), dim3(256), 0, stream, jobs, njobs, pass, steps,
-                           st);
+        hipLaunchKernelGGL(k_tau_batch, dim3(div_up(total_sites, block)), dim3(block), 0, stream, jobs, njobs, pass,
+                           steps, st);
     return launch_status("tau_diffuse_adam_batch");
 }
 

@@ -73,3 +73,38 @@ def test_module_map_imports_the_headers(tmp_path):
     r = subprocess.run([clang, "-fmodules", f"-fmodules-cache-path={tmp_path}", "-fsyntax-only", "-I", inc, str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the option and result structs (sad_amd/_lib.py) have the header's size and field
+    offsets, compiled from include/sad_hip.h by the host C compiler."""
+    import os
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    structs = {"SadhFitOptions": _lib.FitOptions, "SadhFitResult": _lib.FitResult, "SadhBatchJob": _lib.BatchJob,
+               "SadhBatchResult": _lib.BatchResult, "SadhServerOptions": _lib.ServerOptions}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sad_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            parts = line.split()
+            got[(parts[0], parts[1])] = int(parts[2])
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

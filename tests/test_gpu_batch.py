@@ -171,3 +171,21 @@ def test_masked_batch_matches_single_fits():
         g = trainer.fit(t, s0, opts, mask=masks[i])
         assert_same_fit(out["jobs"][i], g)
         assert out["jobs"][i]["n_logs"] == 5
+
+
+def test_autotune_keeps_every_result():
+    """autotune times steady ticks (from the 41st) under each launch schedule (128- or 64-slot tile hash,
+    256- or 64-thread tau blocks) and keeps the fastest; no schedule changes a result, so every job ends as in
+    the batch without tuning, with and without graph replays."""
+    specs = [(150, 100, 62, 500), (90, 170, 63, 400), (64, 48, 64, 120)]
+    tg, st = _jobs(specs)
+    opts = config.fit_options(iterations=90, target_bpp=1.5, densify_freq=20, prune_freq=20, densify_start=20,
+                              prune_start=40, max_sites=2000, log_freq=30)
+    plain = trainer.fit_batch(tg, st, opts)
+    assert plain["tuned_hash_slots"] == 0 and plain["tuned_tau_block"] == 0
+    for graph in (0, 1):
+        tuned = trainer.fit_batch(tg, st, dict(opts, autotune=1, graph_ticks=graph))
+        print("graph", graph, "tuned schedule: hash", tuned["tuned_hash_slots"], "tau block", tuned["tuned_tau_block"])
+        assert tuned["tuned_hash_slots"] in (64, 128) and tuned["tuned_tau_block"] in (64, 256)
+        for a, b in zip(plain["jobs"], tuned["jobs"]):
+            assert_same_fit(b, a)
