@@ -62,7 +62,13 @@ SAD_HD float sad_exp_core(float xc)
     const float LN2_LO = 0x1.7f7d1cp-20f;  /* ln2 - head */
     const float SHIFTER = 12582912.0f;                     /* 1.5 * 2^23 */
     float t = xc * L2E;
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* v_rndne_f32: the same round-to-nearest-even of t in one instruction */
+    float kf = __builtin_rintf(t);
+    (void)SHIFTER;
+#else
     float kf = (t + SHIFTER) - SHIFTER;          /* round-to-nearest-even of t */
+#endif
     float r = fmaf(-kf, LN2_HI, xc);
     r = fmaf(-kf, LN2_LO, r);
     float p = 0x1.a01a02p-13f;   /* 1/5040 */
@@ -74,9 +80,16 @@ SAD_HD float sad_exp_core(float xc)
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     int k = (int)kf;
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* v_ldexp_f32 scales by 2^k with one IEEE rounding (also into the
+     * subnormal range), which is the once-rounded product below; over
+     * every float input both forms give the same bits on gfx950 */
+    return __builtin_amdgcn_ldexpf(p, k);
+#else
     int k1 = k >> 1;                             /* arithmetic shift: floor(k / 2), at least -75 */
     int k2 = k - k1;
     return (p * sad_pow2i(k1)) * sad_pow2i(k2);
+#endif
 }
 
 SAD_HD float sad_expf(float x)
