@@ -49,14 +49,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_SOURCE = os.path.join(ROOT, "sad_amd", "csrc", "k_pixel.hip")
+# the gradient kernel's source and every header it includes
+KERNEL_SOURCES = [os.path.join(ROOT, *p.split("/")) for p in (
+    "sad_amd/csrc/k_pixel.hip", "sad_amd/csrc/fit_common.h", "sad_amd/csrc/batch.h", "sad_amd/csrc/sad_device.h",
+    "sad_amd/csrc/sad_math.h", "include/sad_hip.h", "include/sad_types.h")]
 
 
 def kernel_source_sha() -> str:
-    """first 16 hex digits of the SHA-256 of the gradient kernel's source"""
+    """first 16 hex digits of the SHA-256 of the gradient kernel's source files, concatenated in a fixed order"""
     import hashlib
-    with open(KERNEL_SOURCE, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:16]
+    h = hashlib.sha256()
+    for path in KERNEL_SOURCES:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def profile_value(name: str, key: str):
