@@ -194,8 +194,9 @@ struct PixelArgs {
     const float* grad_out;     /* BWD */
     float* out;                /* RENDER: [H][W][3] */
     uint32_t cells;            /* RENDER: 1 = each site's colour is sad_hash_color of its id (cell picture) */
-    /* GRAD: the first list_blocks blocks of the launch write the tau passes' neighbour lists (tau_list_only)
-     * instead of a tile; they read only sites and the candidate field, which the gradient does not change */
+    /* GRAD: the last list_blocks blocks of the launch write the tau passes' neighbour lists (tau_list_only)
+     * instead of a tile; they read only sites and the candidate field, which the gradient does not change.
+     * Placed after the tiles, these short blocks fill the slots that the last tiles leave. */
     uint4* nbr;
     float nbr_lambda, nbr_inv_ds;
     uint32_t list_blocks;
@@ -699,15 +700,15 @@ __global__ __launch_bounds__(256) void k_pixel(PixelArgs A)
 {
     __shared__ PixelLds<MODE, REMOVAL, H> L;
     if constexpr (MODE == MODE_GRAD) {
-        if (blockIdx.x < A.list_blocks) {
-            const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+        if (blockIdx.x >= A.ntiles) {
+            const uint32_t i = (blockIdx.x - A.ntiles) * 256u + threadIdx.x;
             if (i < A.n)
                 tau_list_only(i, reinterpret_cast<const uint4*>(A.cand0), reinterpret_cast<const uint4*>(A.cand1),
                               A.sites, A.n, A.nbr_lambda, A.cw, A.ch, A.nbr_inv_ds, A.nbr);
             return;
         }
     }
-    pixel_tile<MODE, REMOVAL, H, DST>(A, xcd_tile(blockIdx.x - A.list_blocks, A.ntiles), L);
+    pixel_tile<MODE, REMOVAL, H, DST>(A, xcd_tile(blockIdx.x, A.ntiles), L);
 }
 
 /* Batched fit: one launch over the concatenated tiles of every job. */
@@ -986,8 +987,7 @@ int launch_gradients_fix(const uint32_t* cand0, const uint32_t* cand1, const flo
     A.iss = iss; A.n = n;
     A.width = width; A.height = height; A.cw = cw; A.ch = ch;
     if (nbr_lists && n > 0u) {
-        /* a multiple of 8 blocks keeps the tiles' XCD grouping aligned */
-        A.list_blocks = div_up(div_up(n, 256u), 8u) * 8u;
+        A.list_blocks = div_up(n, 256u);
         A.nbr = reinterpret_cast<uint4*>(nbr_lists);
         A.nbr_lambda = nbr_lambda;
         A.nbr_inv_ds = 1.0f / fmaxf((float)ds, 1.0f);
