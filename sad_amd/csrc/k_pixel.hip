@@ -384,7 +384,10 @@ __device__ __forceinline__ void pixel_tile(const PixelArgs& A, const uint32_t ti
     float sum = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        w[k] = (valid[k] && has) ? sad_expf_nonpos(logit[k] - mx) : 0.0f;   /* logit - max <= 0 */
+        /* evaluated for every rank and then selected, so the eight exps share one basic block (an invalid rank
+         * gives 0 or NaN here, which the select drops) */
+        const float e = sad_expf_nonpos(logit[k] - mx);   /* logit - max <= 0 */
+        w[k] = (valid[k] && has) ? e : 0.0f;
         sum += w[k];
     }
     const float inv = 1.0f / fmaxf(sum, 1e-8f);
